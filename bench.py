@@ -1,0 +1,240 @@
+"""Benchmark: FP64 stiffness operator apply y = K x through the dofmap.
+
+Workload: structured 48^3 hexahedral mesh (110592 cells), degree P=6 Lagrange
+on GLL nodes, Q=7 Gauss-Legendre points per direction, int32 dofmap with
+shared faces (289^3 = 24,137,569 global dofs), per-cell affine Jacobians
+J = I + 0.1 U(-1,1) with G = detJ w J^-1 J^-T per point in FP64, x ~ U[-1,1].
+
+One step = one continuous operator application: clear y, then the fused
+gather -> contraction chain -> geometry -> transposed chain -> atomic scatter
+kernel.  Inputs are resident in HBM before the timed region starts.
+
+Prints ONE JSON line (rank 0).  ``value`` is global dofs per second (GDoF/s).
+``roofline`` prices the fused apply kernel against the MI355X HBM peak using
+the algorithmic bytes (x + y + dofmap + G once each), with the kernel's own
+duration taken from HIP events on the stream it runs on.  ``cpu_baseline``
+times the reference's own stiffness kernel (src/stiffness.hpp:34-99, built
+with -Ofast from the reference sources; oracle/_ref) on the host cores over
+a bounded sample of cells of the same shape.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import threading
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "GDoF/s stiffness apply P=6 FP64 vs BF16-MFMA+refine @1e-10; p99 solve ms"
+HBM_PEAK_GBS = 8000.0
+
+
+def _uniform(seed, n, lo, hi):
+    """Counter-based SplitMix64 uniforms (same generator as the oracle)."""
+    g = np.uint64(0x9E3779B97F4A7C15)
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed) + np.arange(1, n + 1, dtype=np.uint64) * g
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    u = (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+    return lo + (hi - lo) * u
+
+
+def cpu_baseline(P: int, ncells_global: int, ndofs_global: int,
+                 budget_s: float = 12.0) -> dict:
+    """Reference stiffness kernel on the host cores, one thread per core on
+    disjoint cell ranges (ctypes releases the GIL), 10 applies, mean of the
+    last 5 (bench_stiffness.cpp:95-106).  Reported as global-dof throughput
+    of the same mesh: cells/s * (ndofs_global / ncells_global)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O  # checker / baseline infrastructure only
+
+    variant = O.baseline_variant()
+    if not O.ref_available(variant):
+        return {"value": None, "unit": "GDoF/s", "cores": 0,
+                "kind": "reference", "sample": "oracle/_ref not built"}
+    L = O.ref_lib(variant)
+    N = P + 1
+    nd = N ** 3
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    cores = max(1, min(cores, 16))
+    phi2 = np.concatenate([O.basis_tables(P, N)])  # Q = P+1 as the reference
+    # pick the fastest loop order / block on one core (run.sh:13-23 sweep)
+    probe = 256
+    U = _uniform(5, probe * nd, -1, 1)
+    G = _uniform(6, probe * nd * 6, 0, 1)
+    W = np.zeros(probe * nd)
+    best = None
+    for order in range(6):
+        for (mb, nb) in ((0, 0), (4, 16), (8, 8), (0, 16)):
+            t0 = time.perf_counter()
+            rc = L.ref_baseline_stiffness(P, order, mb, nb, phi2, U, W, G, probe)
+            dt = time.perf_counter() - t0
+            if rc == 0 and (best is None or dt < best[0]):
+                best = (dt, order, mb, nb)
+    _, order, mb, nb = best
+    per_cell = best[0] / probe
+    # bounded sample: ~budget_s of CPU time in total over 10 applies
+    per_thread = int(max(64, min(8192, budget_s / (10 * per_cell))))
+    bufs = []
+    for t in range(cores):
+        bufs.append((_uniform(10 + t, per_thread * nd, -1, 1),
+                     np.zeros(per_thread * nd),
+                     _uniform(50 + t, per_thread * nd * 6, 0, 1)))
+    times = []
+    for it in range(10):
+        barrier = threading.Barrier(cores + 1)
+        done = threading.Barrier(cores + 1)
+
+        def work(t):
+            u, w, g = bufs[t]
+            barrier.wait()
+            L.ref_baseline_stiffness(P, order, mb, nb, phi2, u, w, g, per_thread)
+            done.wait()
+
+        ths = [threading.Thread(target=work, args=(t,)) for t in range(cores)]
+        for th in ths:
+            th.start()
+        barrier.wait()
+        t0 = time.perf_counter()
+        done.wait()
+        times.append(time.perf_counter() - t0)
+        for th in ths:
+            th.join()
+    elapsed = float(np.mean(times[5:]))
+    cells_per_s = cores * per_thread / elapsed
+    names = ("ijk", "ikj", "jik", "jki", "kij", "kji")
+    return {
+        "value": cells_per_s * ndofs_global / ncells_global / 1e9,
+        "unit": "GDoF/s",
+        "cores": cores,
+        "kind": "reference",
+        "sample": (f"reference stiffness_operator<double,P={P}> "
+                   f"(src/stiffness.hpp:34-99, -Ofast {variant}), "
+                   f"{cores} threads x {per_thread} cells, order={names[order]} "
+                   f"Mb={mb} Nb={nb}, mean of applies 5-9 of 10 = "
+                   f"{elapsed * 1e3:.1f} ms; {cells_per_s:.4g} cells/s scaled "
+                   f"to the mesh's global dofs per cell"),
+        "cell_local_gdofs": cells_per_s * nd / 1e9,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=48, help="cells per direction")
+    ap.add_argument("--degree", type=int, default=6)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {a.gpus} is not supported "
+              "(the cell loop shards by contiguous cell ranges, see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import femtc_amd as F
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    n, P = a.n, a.degree
+    Q = P + 1
+    N = P + 1
+    C = n ** 3
+    ng = (n * P + 1) ** 3
+    print(f"[bench] mesh {n}^3 P={P} Q={Q} cells={C} dofs={ng}", file=sys.stderr,
+          flush=True)
+    stream = torch.cuda.current_stream()
+    tab = torch.from_numpy(F.basis_tables(P, Q)).to(dev)
+    J = _uniform(3, 9 * C, -1, 1) * 0.1
+    J = (J.reshape(C, 3, 3) + np.eye(3)).ravel()
+    dJ = torch.from_numpy(J).to(dev)
+    G = torch.empty(C * Q ** 3 * 6, device=dev, dtype=torch.float64)
+    F.geometry_affine(dJ, Q, G=G)
+    dm = torch.empty(C * N ** 3, device=dev, dtype=torch.int32)
+    F.dofmap_structured(n, n, n, P, dm)
+    x = torch.from_numpy(_uniform(4, ng, -1, 1)).to(dev)
+    y = torch.empty(ng, device=dev, dtype=torch.float64)
+    del dJ
+    torch.cuda.synchronize()
+
+    def step():
+        y.zero_()
+        F.global_apply(F.OP_STIFFNESS, tab, x, y, dm, C, P, Q, G=G,
+                       accumulate=True)
+
+    for _ in range(a.warmup):
+        step()
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True),
+           torch.cuda.Event(enable_timing=True)) for _ in range(a.steps)]
+    t0 = time.perf_counter()
+    for i in range(a.steps):
+        y.zero_()
+        ev[i][0].record(stream)
+        F.global_apply(F.OP_STIFFNESS, tab, x, y, dm, C, P, Q, G=G,
+                       accumulate=True)
+        ev[i][1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kern_ms = float(np.mean([s.elapsed_time(e) for s, e in ev]))
+    ms_per_step = wall * 1e3 / a.steps
+    value = ng / (ms_per_step * 1e-3) / 1e9
+
+    alg_bytes = 8 * ng * 2 + 4 * C * N ** 3 + 8 * 6 * C * Q ** 3
+    achieved = alg_bytes / (kern_ms * 1e-3) / 1e9
+    # a sanity check of the result: y^T 1 = x^T K 1 = 0 up to rounding
+    s = float(y.sum().item())
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "GDoF/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"stiffness apply y=Kx, {n}^3 hex mesh, P={P} GLL, "
+                         f"Q={Q} Gauss, int32 dofmap gather/scatter, FP64"),
+            "cells": C, "global_dofs": ng, "degree": P, "nq": Q,
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel": f"apply_cells<double,{N},{Q},OP_STIFF,gather>",
+            "kernel_ms": kern_ms,
+            "algorithmic_bytes": alg_bytes,
+        },
+        "cell_local_gdofs": C * N ** 3 / (ms_per_step * 1e-3) / 1e9,
+        "check_sum_y": s,
+    }
+    if not a.no_cpu_baseline:
+        print("[bench] cpu baseline", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(P, C, ng)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,21 @@
+"""femtc_amd -- MI355X (gfx950) sum-factorized mass / stiffness operators.
+
+Host-side mirror of the reference's operator interface over the C ABI of
+libfemtc.so (include/femtc.h).  Importing the package does not load the
+library; the first call does, and fails loudly if it is not built.
+"""
+from ._lib import (BF16, F32, F64, MODE_CORRECT, MODE_REFERENCE, OP_KPM,
+                   OP_MASS, OP_STIFFNESS, FemtcError, lib)
+from .linalg import Order, string2order
+from .mesh import (basis_tables, dofmap_structured, gauss_rule, geometry_affine,
+                   gll_points)
+from .operators import (global_apply, kpm_apply, mass_apply, mass_operator,
+                        stiffness_apply, stiffness_operator)
+
+__all__ = [
+    "BF16", "F32", "F64", "MODE_CORRECT", "MODE_REFERENCE", "OP_KPM", "OP_MASS",
+    "OP_STIFFNESS", "FemtcError", "lib", "Order", "string2order",
+    "basis_tables", "dofmap_structured", "gauss_rule", "geometry_affine",
+    "gll_points", "global_apply", "kpm_apply", "mass_apply", "mass_operator",
+    "stiffness_apply", "stiffness_operator",
+]

@@ -1,0 +1,98 @@
+"""Loader for libfemtc.so, the HIP (gfx950) library behind the C ABI in
+include/femtc.h.  There is no CPU fallback: if the library is missing this
+module raises, so a GPU run can never silently compute elsewhere."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from functools import lru_cache
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libfemtc.so")
+
+OK = 0
+ERR_INVALID_ARGUMENT = 1
+ERR_DEGREE_NOT_SUPPORTED = 2
+ERR_ORDER_NOT_SUPPORTED = 3
+ERR_BLOCK_NOT_SUPPORTED = 4
+ERR_INVALID_ORDER_STRING = 5
+ERR_HIP = 6
+ERR_OUT_OF_MEMORY = 7
+ERR_NOT_CONVERGED = 8
+ERR_BREAKDOWN = 9
+ERR_NO_DEVICE = 10
+
+F64, F32, BF16 = 0, 1, 2
+MODE_CORRECT, MODE_REFERENCE = 0, 1
+OP_MASS, OP_STIFFNESS, OP_KPM = 0, 1, 2
+
+_vp = C.c_void_p
+_i32 = C.c_int
+_i64 = C.c_int64
+_sz = C.c_size_t
+_dp = C.POINTER(C.c_double)
+
+# name -> (restype, argtypes); the exported surface of include/femtc.h
+SIGNATURES = {
+    "femtc_status_string": (C.c_char_p, [_i32]),
+    "femtc_last_error": (C.c_char_p, []),
+    "femtc_version": (_i32, []),
+    "femtc_max_degree": (_i32, []),
+    "femtc_string2order": (_i32, [C.c_char_p, C.POINTER(_i32)]),
+    "femtc_check_ladder": (_i32, [_i32, _i32, _i32, _i32]),
+    "femtc_gll_points": (_i32, [_i32, _dp]),
+    "femtc_gauss_rule": (_i32, [_i32, _dp, _dp]),
+    "femtc_basis_tables": (_i32, [_i32, _i32, _dp]),
+    "femtc_mass_apply": (_i32, [_i32, _i32, _i32, _i32, _vp, _vp, _vp, _vp,
+                                _i64, _vp]),
+    "femtc_stiffness_apply": (_i32, [_i32, _i32, _i32, _i32, _vp, _vp, _vp,
+                                     _vp, _i64, _vp]),
+    "femtc_kpm_apply": (_i32, [_i32, _i32, _i32, _vp, _vp, _vp, _vp, _vp,
+                               _i64, _vp]),
+    "femtc_global_apply": (_i32, [_i32, _i32, _i32, _i32, _vp, _vp, _vp, _i64,
+                                  _vp, _vp, _vp, _i64, _vp]),
+    "femtc_global_apply_add": (_i32, [_i32, _i32, _i32, _i32, _vp, _vp, _vp,
+                                      _i64, _vp, _vp, _vp, _i64, _vp]),
+    "femtc_mass_operator": (_i32, [_i32, _vp, _sz, _vp, _vp, _vp, _sz, _i32,
+                                   _i32, _i32, _i32, _i32]),
+    "femtc_stiffness_operator": (_i32, [_i32, _vp, _sz, _vp, _vp, _vp, _i32,
+                                        _i32, _i32, _i32, _i32, _i32]),
+    "femtc_dofmap_structured": (_i32, [_i32, _i32, _i32, _i32, _vp, _vp]),
+    "femtc_geometry_affine": (_i32, [_i64, _i32, _vp, _vp, _vp, _i32, _vp]),
+}
+
+
+class FemtcError(RuntimeError):
+    def __init__(self, status: int, where: str, detail: str = ""):
+        self.status = status
+        msg = f"{where}: {status_string(status)}"
+        if detail:
+            msg += f" ({detail})"
+        super().__init__(msg)
+
+
+@lru_cache(maxsize=None)
+def lib() -> C.CDLL:
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is not built; run `python -m femtc_amd.build` "
+            "(there is no CPU fallback)")
+    L = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    return L
+
+
+def status_string(status: int) -> str:
+    return lib().femtc_status_string(status).decode()
+
+
+def last_error() -> str:
+    return lib().femtc_last_error().decode()
+
+
+def check(status: int, where: str) -> None:
+    if status != OK:
+        raise FemtcError(status, where, last_error())

@@ -1,0 +1,8 @@
+// Fused cell-operator instances for N = 6 dofs per direction (degree 5),
+// Q in {N, N+1, N+2}, FP64 and FP32, cell-local and gathered.
+#include "apply_kernel.hpp"
+
+namespace femtc
+{
+int launch_apply_n6(const ApplyReq& r) { return dispatch_q<6>(r); }
+} // namespace femtc

@@ -1,0 +1,58 @@
+// Shared definitions for the femtc HIP library (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "femtc.h"
+
+namespace femtc
+{
+
+// Operator selectors used by the fused cell kernel.
+enum Op : int
+{
+  OP_MASS = 0,       // (Phi^T x Phi^T x Phi^T) diag(detJw) (Phi x Phi x Phi)
+  OP_STIFF = 1,      // sum_ij B_i^T S_ij B_j
+  OP_KPM = 2,        // stiffness + mass
+  OP_STIFF_LIT = 3,  // the reference's literal stiffness (src/stiffness.hpp:34-99)
+};
+
+// One operator application request, filled by the C-ABI layer and consumed
+// by the per-degree launchers.
+struct ApplyReq
+{
+  int op = OP_MASS;
+  int dtype = FEMTC_F64;
+  int N = 0, Q = 0;
+  int reference_mode = 0;  // backward tables read untransposed (literal)
+  int gather = 0;          // x/y are global vectors addressed through dofmap
+  const void* tab = nullptr;
+  const void* x = nullptr;
+  void* y = nullptr;
+  const void* G = nullptr;
+  const void* detJw = nullptr;
+  const int32_t* dofmap = nullptr;
+  int64_t num_cells = 0;
+  hipStream_t stream = nullptr;
+};
+
+using ApplyLauncher = int (*)(const ApplyReq&);
+
+// Launchers, one translation unit per N = P+1 (apply_n<N>.hip).
+int launch_apply_n2(const ApplyReq&);
+int launch_apply_n3(const ApplyReq&);
+int launch_apply_n4(const ApplyReq&);
+int launch_apply_n5(const ApplyReq&);
+int launch_apply_n6(const ApplyReq&);
+int launch_apply_n7(const ApplyReq&);
+int launch_apply_n8(const ApplyReq&);
+int launch_apply_n9(const ApplyReq&);
+
+inline int hip_status(hipError_t e)
+{
+  return e == hipSuccess ? FEMTC_OK : FEMTC_ERR_HIP;
+}
+
+} // namespace femtc

@@ -1,0 +1,48 @@
+"""1-D tables, structured dofmaps and affine geometry (host tables, device
+meshes)."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from .operators import _dtype_code, _ptr, _stream
+
+
+def gll_points(degree: int) -> np.ndarray:
+    x = np.zeros(degree + 1)
+    _lib.check(_lib.lib().femtc_gll_points(
+        degree, x.ctypes.data_as(_lib._dp)), "gll_points")
+    return x
+
+
+def gauss_rule(nq: int):
+    x, w = np.zeros(nq), np.zeros(nq)
+    _lib.check(_lib.lib().femtc_gauss_rule(
+        nq, x.ctypes.data_as(_lib._dp), w.ctypes.data_as(_lib._dp)),
+        "gauss_rule")
+    return x, w
+
+
+def basis_tables(degree: int, nq: int) -> np.ndarray:
+    """[phi | dphi], each [N][nq], GLL Lagrange basis at Gauss points."""
+    t = np.zeros(2 * (degree + 1) * nq)
+    _lib.check(_lib.lib().femtc_basis_tables(
+        degree, nq, t.ctypes.data_as(_lib._dp)), "basis_tables")
+    return t
+
+
+def dofmap_structured(nx: int, ny: int, nz: int, degree: int, out,
+                      stream=None) -> None:
+    """Fill the int32 device tensor ``out`` [nx*ny*nz*(P+1)^3]."""
+    _lib.check(_lib.lib().femtc_dofmap_structured(
+        nx, ny, nz, degree, _ptr(out), _stream(stream)), "dofmap_structured")
+
+
+def geometry_affine(J, nq: int, G=None, detJw=None, stream=None) -> None:
+    """Per-cell Jacobians J (FP64 device tensor [C*9]) -> G / detJw."""
+    ref = G if G is not None else detJw
+    _lib.check(_lib.lib().femtc_geometry_affine(
+        J.numel() // 9, nq, _ptr(J), _ptr(G), _ptr(detJw), _dtype_code(ref),
+        _stream(stream)), "geometry_affine")

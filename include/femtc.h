@@ -1,0 +1,187 @@
+/*
+ * femtc.h -- C ABI of the MI355X (gfx950) sum-factorized finite element
+ * operator library (libfemtc.so).
+ *
+ * Plain pointers and sizes only; no HIP or C++ types cross this boundary.
+ * "stream" arguments are hipStream_t values passed as void* (NULL = the
+ * default stream).  Device entry points are asynchronous and stream-ordered;
+ * host entry points are synchronous.  Every function returns a femtc_status;
+ * nothing is thrown across the boundary.  femtc_last_error() returns a
+ * human-readable detail string for the calling thread's last failure.
+ *
+ * Layouts (the reference's, generalised to Q quadrature points per
+ * direction; src/mass.hpp:25-29, src/stiffness.hpp:44-57):
+ *   N = degree+1, nd = N^3 dofs per cell, nq3 = Q^3 points per cell.
+ *   U, W      [num_cells][nd], row-major [a][b][c], a slowest.
+ *   tables    [phi | dphi], each [N][Q] row-major: phi[d*Q+q] = phi_d(x_q).
+ *   detJw     [num_cells][nq3].
+ *   G         [num_cells][nq3][6] packed (S22,S12,S02,S11,S01,S00) of the
+ *             symmetric 3x3 tensor acting on (d/da, d/db, d/dc)
+ *             (src/stiffness.hpp:25-27).
+ *   dofmap    int32 [num_cells][nd], global dof of each cell dof.
+ */
+#ifndef FEMTC_H
+#define FEMTC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FEMTC_VERSION 1
+#define FEMTC_MAX_DEGREE 8
+
+typedef enum femtc_status
+{
+  FEMTC_OK = 0,
+  FEMTC_ERR_INVALID_ARGUMENT = 1,
+  FEMTC_ERR_DEGREE_NOT_SUPPORTED = 2, /* src/mass.hpp:134-136 */
+  FEMTC_ERR_ORDER_NOT_SUPPORTED = 3,  /* src/mass.hpp:165-167 */
+  FEMTC_ERR_BLOCK_NOT_SUPPORTED = 4,  /* src/mass.hpp:190-192, 215-217 */
+  FEMTC_ERR_INVALID_ORDER_STRING = 5, /* src/linalg.hpp:36 */
+  FEMTC_ERR_HIP = 6,
+  FEMTC_ERR_OUT_OF_MEMORY = 7,
+  FEMTC_ERR_NOT_CONVERGED = 8,
+  FEMTC_ERR_BREAKDOWN = 9,
+  FEMTC_ERR_NO_DEVICE = 10
+} femtc_status;
+
+typedef enum femtc_dtype
+{
+  FEMTC_F64 = 0,
+  FEMTC_F32 = 1,
+  FEMTC_BF16 = 2
+} femtc_dtype;
+
+/* linalg::Order, same enumerators in the same order (src/linalg.hpp:11-19).
+ * The GPU kernels do not depend on it; it is validated and accepted. */
+typedef enum femtc_order
+{
+  FEMTC_ORDER_IJK = 0,
+  FEMTC_ORDER_IKJ = 1,
+  FEMTC_ORDER_JIK = 2,
+  FEMTC_ORDER_JKI = 3,
+  FEMTC_ORDER_KIJ = 4,
+  FEMTC_ORDER_KJI = 5
+} femtc_order;
+
+/* FEMTC_MODE_CORRECT: the mathematically intended operator (true transposed
+ * tables, all gradient chains summed).  FEMTC_MODE_REFERENCE: what the
+ * reference computes, defects included (the backward table is the forward
+ * one, src/mass.hpp:20-21; only the z backward chain of the stiffness
+ * operator survives and its forward "z" field is not differentiated,
+ * src/stiffness.hpp:76-97).  Reference mode requires nq == degree+1. */
+typedef enum femtc_mode
+{
+  FEMTC_MODE_CORRECT = 0,
+  FEMTC_MODE_REFERENCE = 1
+} femtc_mode;
+
+typedef enum femtc_op
+{
+  FEMTC_OP_MASS = 0,
+  FEMTC_OP_STIFFNESS = 1,
+  FEMTC_OP_KPM = 2 /* stiffness + mass */
+} femtc_op;
+
+/* ---------------------------------------------------------------------- */
+/* Status and version                                                      */
+/* ---------------------------------------------------------------------- */
+const char* femtc_status_string(int status);
+const char* femtc_last_error(void);
+int femtc_version(void);
+/* Highest supported degree for the given quadrature offset (nq - degree - 1
+ * in {0,1,2}); degrees 1..FEMTC_MAX_DEGREE are supported. */
+int femtc_max_degree(void);
+
+/* linalg::string2order (src/linalg.hpp:21-37): "ijk".."kji" -> *order;
+ * anything else returns FEMTC_ERR_INVALID_ORDER_STRING. */
+int femtc_string2order(const char* name, int* order);
+
+/* Validate the CPU blocking knobs of the reference dispatch ladder
+ * (src/mass.hpp:140-219): order in 0..5, Mb and Nb in {0,4,8,16}. */
+int femtc_check_ladder(int degree, int order, int Mb, int Nb);
+
+/* ---------------------------------------------------------------------- */
+/* 1-D tables on the reference interval [0,1] (host, FP64)                 */
+/* ---------------------------------------------------------------------- */
+/* Gauss-Lobatto-Legendre nodes, degree+1 values, increasing. */
+int femtc_gll_points(int degree, double* x);
+/* Gauss-Legendre points and weights (weights sum to 1), nq values each. */
+int femtc_gauss_rule(int nq, double* x, double* w);
+/* [phi | dphi] of the GLL Lagrange basis at the Gauss points, 2*N*nq values. */
+int femtc_basis_tables(int degree, int nq, double* tables);
+
+/* ---------------------------------------------------------------------- */
+/* Cell-local operators on device buffers (stream-ordered)                 */
+/* dtype FEMTC_F64 or FEMTC_F32: every array is of that type.              */
+/* ---------------------------------------------------------------------- */
+/* W = M U per cell. Replaces operators::mass_operator (src/mass.hpp:10-49). */
+int femtc_mass_apply(int dtype, int degree, int nq, int mode,
+                     const void* tables, const void* U, void* W,
+                     const void* detJw, int64_t num_cells, void* stream);
+/* W = K U per cell. Replaces operators::stiffness_operator
+ * (src/stiffness.hpp:34-99). */
+int femtc_stiffness_apply(int dtype, int degree, int nq, int mode,
+                          const void* tables, const void* U, void* W,
+                          const void* G, int64_t num_cells, void* stream);
+/* W = (K + M) U per cell (correct mode only). */
+int femtc_kpm_apply(int dtype, int degree, int nq, const void* tables,
+                    const void* U, void* W, const void* G, const void* detJw,
+                    int64_t num_cells, void* stream);
+
+/* ---------------------------------------------------------------------- */
+/* Continuous operator: y = A x through the dofmap (y overwritten).        */
+/* op is a femtc_op; G is needed for stiffness/K+M, detJw for mass/K+M.    */
+/* Contributions to shared dofs are summed with device atomics, so the     */
+/* last bits of y may vary from run to run.                                */
+/* ---------------------------------------------------------------------- */
+int femtc_global_apply(int op, int dtype, int degree, int nq,
+                       const void* tables, const void* x, void* y,
+                       int64_t ndofs, const int32_t* dofmap, const void* G,
+                       const void* detJw, int64_t num_cells, void* stream);
+/* y += A x: the same without clearing y first (y must be initialised). */
+int femtc_global_apply_add(int op, int dtype, int degree, int nq,
+                           const void* tables, const void* x, void* y,
+                           int64_t ndofs, const int32_t* dofmap, const void* G,
+                           const void* detJw, int64_t num_cells, void* stream);
+
+/* ---------------------------------------------------------------------- */
+/* Reference-shaped synchronous entry points on HOST buffers: the C side   */
+/* of the operators::mass_operator<T> / stiffness_operator<T> shims        */
+/* (src/mass.hpp:196-199, src/stiffness.hpp:245-248).  Q = degree+1 as in  */
+/* the reference.  phi_len is the number of table values the caller owns:  */
+/* (P+1)^2 for mass, 2 (P+1)^2 for stiffness ([phi | dphi]).  Copies in,   */
+/* applies on the current device, copies W back.                           */
+/* ---------------------------------------------------------------------- */
+int femtc_mass_operator(int dtype, const void* phi, size_t phi_len,
+                        const void* U, void* W, const void* detJ,
+                        size_t num_cells, int degree, int order, int Mb,
+                        int Nb, int mode);
+int femtc_stiffness_operator(int dtype, const void* phi, size_t phi_len,
+                             const void* U, void* W, const void* G,
+                             int num_cells, int degree, int order, int Mb,
+                             int Nb, int mode);
+
+/* ---------------------------------------------------------------------- */
+/* Mesh and geometry (device, stream-ordered)                              */
+/* ---------------------------------------------------------------------- */
+/* Structured nx*ny*nz hexahedral mesh of degree P: cell (ci,cj,ck) has id
+ * (ci*ny+cj)*nz+ck; global dofs are the lexicographic (nx P+1)(ny P+1)
+ * (nz P+1) grid, first axis slowest. */
+int femtc_dofmap_structured(int nx, int ny, int nz, int degree,
+                            int32_t* dofmap, void* stream);
+/* Per-cell constant Jacobians J [num_cells][9] (FP64, row-major,
+ * J[r][s] = dx_r/dxi_s) -> G [num_cells][nq^3][6] and/or detJw
+ * [num_cells][nq^3] (either may be NULL), computed in FP64 and stored as
+ * dtype (F64 or F32). */
+int femtc_geometry_affine(int64_t num_cells, int nq, const double* J,
+                          void* G, void* detJw, int dtype, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FEMTC_H */

@@ -1,0 +1,103 @@
+"""CPU-side checks of the C ABI: the library loads, exports every function
+include/femtc.h declares, and its host-only logic (tables, string2order,
+ladder validation) agrees with the oracle and the reference fixtures.
+No GPU compute is called here."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import femtc_amd as F
+from femtc_amd import _lib
+import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = np.load(os.path.join(ROOT, "tests", "golden", "ref_goldens.npz"))
+
+
+def _declared():
+    text = open(os.path.join(ROOT, "include", "femtc.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(femtc_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    names = _declared()
+    assert len(names) >= 15
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH],
+                         capture_output=True, text=True, check=True).stdout
+    exported = set(re.findall(r" T (femtc_\w+)", out))
+    missing = [n for n in names if n not in exported]
+    assert not missing, missing
+    L = F.lib()
+    for n in names:
+        assert hasattr(L, n)
+    # the Python binding table covers the header exactly
+    assert sorted(_lib.SIGNATURES) == names
+
+
+def test_status_strings_and_version():
+    L = F.lib()
+    assert L.femtc_version() == 1
+    assert L.femtc_max_degree() == 8
+    assert _lib.status_string(_lib.ERR_DEGREE_NOT_SUPPORTED) == "degree not supported"
+    assert _lib.status_string(_lib.ERR_BLOCK_NOT_SUPPORTED) == "Block size not supported"
+    assert _lib.status_string(_lib.ERR_INVALID_ORDER_STRING) == "Invalid loop order"
+
+
+def test_string2order_matches_reference():
+    # reference codes from src/linalg.hpp:21-37 (fixture), -1 = throw
+    for name, code in zip(GOLD["s2o_names"], GOLD["s2o_codes"]):
+        if code < 0:
+            with pytest.raises(RuntimeError, match="Invalid loop order"):
+                F.string2order(str(name))
+        else:
+            assert F.string2order(str(name)) == code
+            assert F.Order(code).name == str(name)
+
+
+def test_ladder_validation_matches_reference_messages():
+    L = F.lib()
+    for (deg, order, Mb, Nb), msg in zip(GOLD["ladder_cases"], GOLD["ladder_msgs"]):
+        st = L.femtc_check_ladder(int(deg), int(order), int(Mb), int(Nb))
+        assert _lib.status_string(st) == str(msg)
+    assert L.femtc_check_ladder(3, 4, 8, 16) == 0
+    assert L.femtc_check_ladder(3, 6, 0, 0) == _lib.ERR_ORDER_NOT_SUPPORTED
+    # degrees the reference instantiates but the GPU build does not
+    assert L.femtc_check_ladder(9, 0, 0, 0) == _lib.ERR_DEGREE_NOT_SUPPORTED
+
+
+@pytest.mark.parametrize("P", [1, 2, 3, 4, 5, 6, 7, 8])
+def test_tables_match_oracle(P):
+    np.testing.assert_allclose(F.gll_points(P), O.gll_points(P), atol=2e-16)
+    for Q in (P + 1, P + 2, P + 3):
+        x, w = F.gauss_rule(Q)
+        xo, wo = O.gl_rule(Q)
+        np.testing.assert_allclose(x, xo, atol=3e-16)
+        np.testing.assert_allclose(w, wo, rtol=1e-14)
+        np.testing.assert_allclose(F.basis_tables(P, Q), O.basis_tables(P, Q),
+                                   rtol=1e-13, atol=1e-13)
+
+
+def test_argument_errors_without_device_work():
+    L = F.lib()
+    # unsupported degree / shape are rejected before any HIP call
+    assert L.femtc_mass_apply(0, 9, 10, 0, None, None, None, None, 10, None) \
+        == _lib.ERR_DEGREE_NOT_SUPPORTED
+    assert L.femtc_mass_apply(0, 3, 3, 0, None, None, None, None, 10, None) \
+        == _lib.ERR_INVALID_ARGUMENT
+    assert L.femtc_stiffness_apply(0, 3, 5, 1, None, None, None, None, 10, None) \
+        == _lib.ERR_INVALID_ARGUMENT  # reference mode needs nq = P+1
+    assert L.femtc_mass_apply(0, 3, 4, 0, None, None, None, None, 0, None) == 0
+    # the stiffness shim needs [phi | dphi]; bench_stiffness.cpp:74 passes N^2
+    phi = np.zeros(16)
+    U = np.zeros(64)
+    W = np.zeros(64)
+    G = np.zeros(64 * 6)
+    st = L.femtc_stiffness_operator(0, phi.ctypes.data, 16, U.ctypes.data,
+                                    W.ctypes.data, G.ctypes.data, 1, 3, 0, 0, 0, 1)
+    assert st == _lib.ERR_INVALID_ARGUMENT
+    assert "dphi" in _lib.last_error()
