@@ -40,6 +40,12 @@ SIGNATURES = {
     "femtc_max_degree": (_i32, []),
     "femtc_string2order": (_i32, [C.c_char_p, C.POINTER(_i32)]),
     "femtc_check_ladder": (_i32, [_i32, _i32, _i32, _i32]),
+    "femtc_device_count": (_i32, [C.POINTER(_i32)]),
+    "femtc_device_alloc": (_i32, [C.POINTER(_vp), _sz]),
+    "femtc_device_free": (_i32, [_vp]),
+    "femtc_memcpy_htod": (_i32, [_vp, _vp, _sz]),
+    "femtc_memcpy_dtoh": (_i32, [_vp, _vp, _sz]),
+    "femtc_device_synchronize": (_i32, []),
     "femtc_gll_points": (_i32, [_i32, _dp]),
     "femtc_gauss_rule": (_i32, [_i32, _dp, _dp]),
     "femtc_basis_tables": (_i32, [_i32, _i32, _dp]),

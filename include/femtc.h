@@ -105,6 +105,16 @@ int femtc_string2order(const char* name, int* order);
 int femtc_check_ladder(int degree, int order, int Mb, int Nb);
 
 /* ---------------------------------------------------------------------- */
+/* Device memory helpers (current HIP device)                              */
+/* ---------------------------------------------------------------------- */
+int femtc_device_count(int* n);
+int femtc_device_alloc(void** ptr, size_t bytes);
+int femtc_device_free(void* ptr);
+int femtc_memcpy_htod(void* dst, const void* src, size_t bytes);
+int femtc_memcpy_dtoh(void* dst, const void* src, size_t bytes);
+int femtc_device_synchronize(void);
+
+/* ---------------------------------------------------------------------- */
 /* 1-D tables on the reference interval [0,1] (host, FP64)                 */
 /* ---------------------------------------------------------------------- */
 /* Gauss-Lobatto-Legendre nodes, degree+1 values, increasing. */
