@@ -5,9 +5,10 @@ on GLL nodes, Q=7 Gauss-Legendre points per direction, int32 dofmap with
 shared faces (289^3 = 24,137,569 global dofs), per-cell affine Jacobians
 J = I + 0.1 U(-1,1) with G = detJ w J^-1 J^-T per point in FP64, x ~ U[-1,1].
 
-One step = one continuous operator application: clear y, then the fused
-gather -> contraction chain -> geometry -> transposed chain -> atomic scatter
-kernel.  Inputs are resident in HBM before the timed region starts.
+One step = one continuous operator application: clear y (fill kernel), then
+the fused gather -> contraction chain -> geometry -> transposed chain ->
+atomic scatter kernel.  Inputs are resident in HBM before the timed region
+starts.
 
 Prints ONE JSON line (rank 0).  ``value`` is global dofs per second (GDoF/s).
 ``roofline`` prices the fused apply kernel against the MI355X HBM peak using
@@ -84,8 +85,10 @@ def cpu_baseline(P: int, ncells_global: int, ndofs_global: int,
                 best = (dt, order, mb, nb)
     _, order, mb, nb = best
     per_cell = best[0] / probe
-    # bounded sample: ~budget_s of CPU time in total over 10 applies
-    per_thread = int(max(64, min(8192, budget_s / (10 * per_cell))))
+    # the whole mesh split evenly over the threads, unless that would exceed
+    # about budget_s of CPU time over the 10 applies
+    per_thread = -(-ncells_global // cores)
+    per_thread = int(max(64, min(per_thread, budget_s / (10 * per_cell))))
     bufs = []
     for t in range(cores):
         bufs.append((_uniform(10 + t, per_thread * nd, -1, 1),
@@ -133,7 +136,9 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    # the first ~20 applies after allocation run slow (clock and memory
+    # warm-up), so the default warmup is generous
+    ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--n", type=int, default=48, help="cells per direction")
     ap.add_argument("--degree", type=int, default=6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -170,22 +175,22 @@ def main():
     del dJ
     torch.cuda.synchronize()
 
-    def step():
-        y.zero_()
+    def apply():
+        # y += K x into the freshly cleared y
         F.global_apply(F.OP_STIFFNESS, tab, x, y, dm, C, P, Q, G=G,
                        accumulate=True)
 
     for _ in range(a.warmup):
-        step()
+        F.fill_zero(y)
+        apply()
     torch.cuda.synchronize()
     ev = [(torch.cuda.Event(enable_timing=True),
            torch.cuda.Event(enable_timing=True)) for _ in range(a.steps)]
     t0 = time.perf_counter()
     for i in range(a.steps):
-        y.zero_()
+        F.fill_zero(y)
         ev[i][0].record(stream)
-        F.global_apply(F.OP_STIFFNESS, tab, x, y, dm, C, P, Q, G=G,
-                       accumulate=True)
+        apply()
         ev[i][1].record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0

@@ -9,12 +9,12 @@ from ._lib import (BF16, F32, F64, MODE_CORRECT, MODE_REFERENCE, OP_KPM,
 from .linalg import Order, string2order
 from .mesh import (basis_tables, dofmap_structured, gauss_rule, geometry_affine,
                    gll_points)
-from .operators import (global_apply, kpm_apply, mass_apply, mass_operator,
-                        stiffness_apply, stiffness_operator)
+from .operators import (fill_zero, global_apply, kpm_apply, mass_apply,
+                        mass_operator, stiffness_apply, stiffness_operator)
 
 __all__ = [
     "BF16", "F32", "F64", "MODE_CORRECT", "MODE_REFERENCE", "OP_KPM", "OP_MASS",
-    "OP_STIFFNESS", "FemtcError", "lib", "Order", "string2order",
+    "OP_STIFFNESS", "FemtcError", "lib", "Order", "string2order", "fill_zero",
     "basis_tables", "dofmap_structured", "gauss_rule", "geometry_affine",
     "gll_points", "global_apply", "kpm_apply", "mass_apply", "mass_operator",
     "stiffness_apply", "stiffness_operator",

@@ -65,6 +65,12 @@ int check_dtype(int dtype)
 
 int run(const ApplyReq& r)
 {
+  // The kernels read a cell's geometry block as pairs of elements.
+  const uintptr_t galign = 2 * (r.dtype == FEMTC_F64 ? 8 : 4);
+  if (r.op != OP_MASS && r.G && (uintptr_t)r.G % galign != 0)
+    return fail(FEMTC_ERR_INVALID_ARGUMENT,
+                "G must be aligned to two elements (16 bytes for FP64, 8 for "
+                "FP32)");
   ApplyLauncher f = launcher_for(r.N);
   if (!f)
     return fail(FEMTC_ERR_DEGREE_NOT_SUPPORTED, "degree not supported");
@@ -302,6 +308,14 @@ int femtc_global_apply_add(int op, int dtype, int degree, int nq,
 {
   return global_apply_impl(op, dtype, degree, nq, tables, x, y, ndofs, dofmap,
                            G, detJw, num_cells, stream, true);
+}
+
+int femtc_fill_zero(void* y, int64_t n, int dtype, void* stream)
+{
+  if (n < 0 || (n > 0 && !y))
+    return fail(FEMTC_ERR_INVALID_ARGUMENT, "invalid fill arguments");
+  const int st = fill_zero(y, n, dtype, (hipStream_t)stream);
+  return st ? fail(st, "fill kernel launch failed") : FEMTC_OK;
 }
 
 // Synchronous host-buffer entry points used by the reference-signature shims.

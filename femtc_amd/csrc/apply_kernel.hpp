@@ -1,34 +1,59 @@
 // Fused sum-factorized cell operator for gfx950.
 //
-// One workgroup applies the operator to CPB cells.  Each cell gets Q*Q
-// threads; a thread owns one "line" of a 3-D cell tensor and performs the
-// 1-D contraction along that line in registers, with the table entries as
-// wave-uniform (scalar) operands.  Between contractions along different axes
-// the partial tensors pass through LDS.  The quadrature-point stage runs in
-// the registers of thread (qa, qb), which owns the Q contiguous points
-// (qa, qb, 0..Q-1), so the geometry stream G[cell][q][6] is read as one
-// contiguous 48*Q-byte run per thread and never staged.
+// Work decomposition.  Each cell gets Q*Q threads; a thread owns one "line"
+// of a 3-D cell tensor and performs the 1-D contraction along that line in
+// registers, with the table entries as wave-uniform (scalar) operands.
+// Between contractions along different axes the partial tensors pass through
+// LDS.  A workgroup is one wave holding CPB = 64/(Q*Q) cells (Q <= 8), or the
+// fewest waves that hold one cell (Q > 8), so the stage barriers are cheap
+// and the workgroups resident on a CU progress independently, hiding each
+// other's memory latency.
 //
-// Contraction schedule (forward: tables read [dof][q]; backward: the
+// Memory access.  Every stream is read with as few, as wide and as
+// contiguous wave instructions as possible (the vector-memory address path
+// is the first limit this kernel hit):
+//  * the geometry G[cell][q][6] of the workgroup's cells is one contiguous
+//    block; it is read at the start of a step as lane-contiguous 16-byte
+//    pieces (8-byte for FP32), kept in registers while the forward
+//    contractions run, and at the quadrature stage written to LDS (over the
+//    then-idle work buffers) and read back per point;
+//  * the gather and the scatter run in thread (a, c), so for a fixed b the
+//    lanes touch N runs of N consecutive global dofs.
+//
+// Axis order.  The forward chain contracts b, then c, then a, so the
+// quadrature-point stage runs in thread (qb, qc), which owns the points
+// (0..Q-1, qb, qc).
+//
+// Contraction schedule (forward tables read [dof][q]; backward tables are the
 // transpose, or the table as stored in reference mode):
-//   F1  axis a  lines (b,c)   : A_p = phi_a U,      A_g = g_a U
-//   F2  axis b  lines (qa,c)  : B_pp = phi_b A_p,   B_pg = g_b A_p,
-//                               B_gp = phi_b A_g
-//   F3  axis c  lines (qa,qb) : u = phi_c B_pp, d_c = g_c B_pp,
-//                               d_b = phi_c B_pg, d_a = phi_c B_gp
+//   F1  axis b  lines (a,c)   : A_p = phi_b U,    A_g = g_b U
+//   F2  axis c  lines (qb,a)  : B_pp = phi_c A_p, B_pg = g_c A_p,
+//                               B_gp = phi_c A_g
+//   F3  axis a  lines (qb,qc) : u = phi_a B_pp,   d_a = g_a B_pp,
+//                               d_c = phi_a B_pg, d_b = phi_a B_gp
 //   pointwise                 : (v_a,v_b,v_c) = S(q) (d_a,d_b,d_c), v_m = detJw u
-//   B3  axis c  lines (qa,qb) : R_a = phi_c' v_a, R_b = phi_c' v_b,
-//                               R_0 = g_c' v_c + phi_c' v_m
-//   B2  axis b  lines (qa,c)  : S_a = phi_b' R_a, S_0 = g_b' R_b + phi_b' R_0
-//   B1  axis a  lines (b,c)   : W = g_a' S_a + phi_a' S_0
-// which is W = sum_ij B_i^T S_ij B_j U (+ B^T detJw B U) with
-// B_a = g(x)phi(x)phi etc.  This is the contraction chain of
-// src/stiffness.hpp:68-97 with the intended transposes and sums; the
-// reference's literal variant (OP_STIFF_LIT) keeps only the chain
+//   B3  axis a  lines (qb,qc) : R_a = g_a' v_a + phi_a' v_m,
+//                               R_b = phi_a' v_b, R_c = phi_a' v_c
+//   B2  axis c  lines (qb,a)  : S_0 = phi_c' R_a + g_c' R_c, S_b = phi_c' R_b
+//   B1  axis b  lines (a,c)   : W = phi_b' S_0 + g_b' S_b
+// which is W = sum_ij B_i^T S_ij B_j U (+ B^T detJw B U), B_a = g(x)phi(x)phi
+// etc.: the contraction chain of src/stiffness.hpp:68-97 with the intended
+// transposes and sums.  The reference's literal variant (OP_STIFF_LIT) keeps
 // W = chain(phi,phi,g)(G2 d_a + G1 d_b + G0 u) (src/stiffness.hpp:76-97).
+//
+// LDS layout.  X[qb][c][a] (F1 -> F2, B2 -> B1) and Y[a][qb][qc] (F2 -> F3,
+// B3 -> B2) with the padded strides of lds_layout.hpp, which keep both lane
+// patterns of each tensor (nearly) free of bank conflicts.
+//
+// Table access.  Every contraction walks the table one dof row at a time
+// (forward: acc[q] += T[d][q] in[d]; backward: out[d] = sum_q T[d][q] in[q]),
+// and each row is fetched with scalar loads pinned inside its iteration, so
+// only a row or two of each table is live in SGPRs instead of the whole
+// 2 N Q table (which does not fit and would spill).
 #pragma once
 
 #include "common.hpp"
+#include "lds_layout.hpp"
 
 namespace femtc
 {
@@ -36,7 +61,7 @@ namespace femtc
 template <int Q>
 constexpr int cells_per_block()
 {
-  return (256 / (Q * Q)) < 1 ? 1 : (256 / (Q * Q));
+  return Q * Q <= 64 ? 64 / (Q * Q) : 1;
 }
 
 template <int Q>
@@ -44,6 +69,12 @@ constexpr int block_threads()
 {
   return ((cells_per_block<Q>() * Q * Q + 63) / 64) * 64;
 }
+
+// A geometry piece: two elements, 16 bytes for FP64 and 8 for FP32.  A
+// cell's G block (6 Q^3 elements, always a multiple of two) is 16-byte
+// aligned for FP64 and 8-byte aligned for FP32.
+template <typename T>
+using Piece = T __attribute__((ext_vector_type(2)));
 
 __device__ inline void atomic_add(double* p, double v)
 {
@@ -57,6 +88,51 @@ __device__ inline T stream_load(const T* p)
   return __builtin_nontemporal_load(p);
 }
 
+// Row d of a [rows][LEN] table, loaded at this point of the program: the
+// offset goes through an empty asm so the compiler cannot hoist the (scalar)
+// loads of every row to the top of the kernel.
+template <int LEN, typename T>
+__device__ inline void load_row(const T* __restrict__ tab, int off, T (&row)[LEN])
+{
+  asm volatile("" : "+s"(off));
+#pragma unroll
+  for (int i = 0; i < LEN; ++i)
+    row[i] = tab[off + i];
+}
+
+// Column d of a table stored [Q][N] (reference mode backward, Q == N).
+template <int LEN, int STRIDE, typename T>
+__device__ inline void load_col(const T* __restrict__ tab, int off, T (&row)[LEN])
+{
+  asm volatile("" : "+s"(off));
+#pragma unroll
+  for (int i = 0; i < LEN; ++i)
+    row[i] = tab[off + i * STRIDE];
+}
+
+template <typename T, int N, int Q, bool REF>
+__device__ inline void load_bwd_row(const T* __restrict__ tab, int d, T (&row)[Q])
+{
+  if constexpr (REF)
+    load_col<Q, N>(tab, d, row);
+  else
+    load_row<Q>(tab, d * Q, row);
+}
+
+// LDS elements of the work area (X, Y and the geometry alias) of one block.
+template <typename T, int N, int Q, int OP>
+constexpr int work_lds_elems()
+{
+  constexpr bool GRAD = OP != OP_MASS;
+  constexpr int CPB = cells_per_block<Q>();
+  constexpr LdsLayout LY = lds_layout<(int)sizeof(T), N, Q>();
+  constexpr int XCELLS = CPB > 1 ? LY.XCELL : (GRAD ? 2 : 1) * Q * LY.XQ;
+  constexpr int YCELLS = CPB > 1 ? LY.YCELL : (GRAD ? 3 : 1) * N * LY.YA;
+  constexpr int WORK = CPB * (XCELLS + YCELLS);
+  constexpr int GELEM = GRAD ? 6 * Q * Q * Q * CPB : 0;
+  return WORK > GELEM ? WORK : GELEM;
+}
+
 template <typename T, int N, int Q, int OP, bool REF, bool GATHER>
 __global__ void __launch_bounds__(block_threads<Q>())
     apply_cells(const T* __restrict__ tab, const T* __restrict__ x,
@@ -65,305 +141,405 @@ __global__ void __launch_bounds__(block_threads<Q>())
                 long long num_cells)
 {
   constexpr int CPB = cells_per_block<Q>();
+  constexpr int BT = block_threads<Q>();
   constexpr int NL = Q * Q;
   constexpr bool LIT = (OP == OP_STIFF_LIT);
   constexpr bool MASSTERM = (OP == OP_MASS || OP == OP_KPM);
   constexpr bool GRAD = (OP != OP_MASS);
-  constexpr bool DC = (OP == OP_STIFF || OP == OP_KPM); // d/dc needed
-  constexpr bool UQ = MASSTERM || LIT;                  // value at q needed
-  constexpr bool CHAIN_AB = DC;                         // R_a/R_b chains
+  constexpr bool FULL = (OP == OP_STIFF || OP == OP_KPM); // all 3 gradients
+  constexpr bool UQ = MASSTERM || LIT;
   constexpr int NX = GRAD ? 2 : 1;
   constexpr int NY = GRAD ? 3 : 1;
-  constexpr int SX = Q * N * N;
-  constexpr int SY = Q * Q * N;
+  // X[qb][c][a] and Y[a][qb][qc] with bank-conflict-minimising strides
+  constexpr LdsLayout LY = lds_layout<(int)sizeof(T), N, Q>();
+  constexpr int XQ = LY.XQ, XC = LY.XC, YQ = LY.YQ, YA = LY.YA;
+  constexpr int SX = Q * XQ; // one X tensor
+  constexpr int SY = N * YA; // one Y tensor
+  constexpr int XCELLS = CPB > 1 ? LY.XCELL : NX * SX;
+  constexpr int YCELLS = CPB > 1 ? LY.YCELL : NY * SY;
+  static_assert(XCELLS >= NX * SX && YCELLS >= NY * SY, "layout fits");
   constexpr int NN = N * N;
   constexpr int N3 = N * N * N;
+  constexpr int Q2 = Q * Q;
   constexpr int Q3 = Q * Q * Q;
+  // geometry block of the workgroup, in pieces of two elements
+  constexpr int GP = GRAD ? 3 * Q3 * CPB : 0;
+  constexpr int GPT = (GP + BT - 1) / BT; // pieces per thread
+  // LDS: X then Y per cell; the geometry block aliases them at the
+  // quadrature stage, when both are idle.
+  constexpr int LDSW = (work_lds_elems<T, N, Q, OP>() + 1) / 2; // pieces
   // LDS slots
   constexpr int X_P = 0, X_G = 1;              // F1 -> F2
   constexpr int Y_PP = 0, Y_PG = 1, Y_GP = 2;  // F2 -> F3
-  constexpr int Y_RA = 0, Y_RB = 1, Y_R0 = NY - 1; // B3 -> B2
-  constexpr int X_SA = 0, X_S0 = NX - 1;       // B2 -> B1
+  constexpr int Y_RA = 0, Y_RB = 1, Y_RC = 2;  // B3 -> B2
+  constexpr int X_S0 = 0, X_SB = 1;            // B2 -> B1
 
-  __shared__ T Xs[CPB * NX * SX];
-  __shared__ T Ys[CPB * NY * SY];
+  __shared__ Piece<T> Ls[LDSW];
+  T* L = reinterpret_cast<T*>(Ls);
 
   const T* __restrict__ ph = tab;
   const T* __restrict__ gr = tab + N * Q;
-  // backward table entry for (dof d, point q)
-  auto bph = [&](int d, int q) -> T { return REF ? ph[q * N + d] : ph[d * Q + q]; };
-  auto bgr = [&](int d, int q) -> T { return REF ? gr[q * N + d] : gr[d * Q + q]; };
 
   const int tid = threadIdx.x;
   const int lc = tid / NL;
   const int ln = tid - lc * NL;
-  const long long cell = (long long)blockIdx.x * CPB + lc;
-  const bool live = (lc < CPB) && (cell < num_cells);
-  T* Xc = Xs + (lc < CPB ? lc : 0) * NX * SX;
-  T* Yc = Ys + (lc < CPB ? lc : 0) * NY * SY;
+  const bool lane_ok = lc < CPB;
+  T* Xc = L + (lane_ok ? lc : 0) * XCELLS;
+  T* Yc = L + CPB * XCELLS + (lane_ok ? lc : 0) * YCELLS;
 
-  // ---------------- global loads issued up front --------------------------
-  T u[N];
-  int32_t idx[GATHER ? N : 1];
-  if (live && ln < NN)
-  {
-    const long long base = cell * N3 + ln;
-#pragma unroll
-    for (int a = 0; a < N; ++a)
-    {
-      if constexpr (GATHER)
-      {
-        idx[a] = dofmap[base + a * NN];
-        u[a] = x[idx[a]];
-      }
-      else
-        u[a] = x[base + a * NN];
-    }
-  }
-  // geometry of the Q points (qa, qb, 0..Q-1) owned by this thread
-  T g6[GRAD ? 6 * Q : 1];
-  T dw[MASSTERM ? Q : 1];
-  if (live)
-  {
-    if constexpr (GRAD)
-    {
-      const T* Gp = G + (cell * Q3 + (long long)ln * Q) * 6;
-#pragma unroll
-      for (int i = 0; i < 6 * Q; ++i)
-        g6[i] = stream_load(Gp + i);
-    }
-    if constexpr (MASSTERM)
-    {
-      const T* Dp = detJw + cell * Q3 + (long long)ln * Q;
-#pragma unroll
-      for (int i = 0; i < Q; ++i)
-        dw[i] = stream_load(Dp + i);
-    }
-  }
+  // thread roles
+  const int r1a = ln / N, r1c = ln - (ln / N) * N;  // F1/B1: (a, c)
+  const int r2q = ln / N, r2a = ln - (ln / N) * N;  // F2/B2: (qb, a)
+  const int yo3 = (ln / Q) * YQ + (ln - (ln / Q) * Q); // F3/B3: (qb, qc)
+  const bool role1 = lane_ok && ln < NN;
+  const bool role2 = lane_ok && ln < N * Q;
+  const int uoff = r1a * NN + r1c; // + b*N within a cell
 
-  // ---------------- F1: contract axis a ------------------------------------
-  if (live && ln < NN)
   {
-#pragma unroll
-    for (int qa = 0; qa < Q; ++qa)
-    {
-      T sp = 0, sg = 0;
-#pragma unroll
-      for (int a = 0; a < N; ++a)
-      {
-        sp = fma(ph[a * Q + qa], u[a], sp);
-        if constexpr (GRAD)
-          sg = fma(gr[a * Q + qa], u[a], sg);
-      }
-      Xc[X_P * SX + qa * NN + ln] = sp;
-      if constexpr (GRAD)
-        Xc[X_G * SX + qa * NN + ln] = sg;
-    }
-  }
-  __syncthreads();
+    const long long cell0 = (long long)blockIdx.x * CPB;
+    const long long cell = cell0 + lc;
+    const bool live = lane_ok && cell < num_cells;
+    const long long ncells_blk =
+        num_cells - cell0 < CPB ? num_cells - cell0 : (long long)CPB;
 
-  // ---------------- F2: contract axis b ------------------------------------
-  if (live && ln < Q * N)
-  {
-    const int qa = ln / N, c = ln - (ln / N) * N;
-    T ap[N], ag[N];
-#pragma unroll
-    for (int b = 0; b < N; ++b)
+    // ---------------- global loads issued up front ------------------------
+    T u[N];
+    int32_t idx[GATHER ? N : 1];
+    if (live && role1)
     {
-      ap[b] = Xc[X_P * SX + qa * NN + b * N + c];
-      if constexpr (GRAD)
-        ag[b] = Xc[X_G * SX + qa * NN + b * N + c];
-    }
-#pragma unroll
-    for (int qb = 0; qb < Q; ++qb)
-    {
-      T spp = 0, spg = 0, sgp = 0;
 #pragma unroll
       for (int b = 0; b < N; ++b)
       {
-        spp = fma(ph[b * Q + qb], ap[b], spp);
-        if constexpr (GRAD)
+        if constexpr (GATHER)
         {
-          spg = fma(gr[b * Q + qb], ap[b], spg);
-          sgp = fma(ph[b * Q + qb], ag[b], sgp);
+          idx[b] = dofmap[cell * N3 + uoff + b * N];
+          u[b] = x[idx[b]];
+        }
+        else
+          u[b] = x[cell * N3 + uoff + b * N];
+      }
+    }
+    Piece<T> gp[GRAD ? GPT : 1];
+    if constexpr (GRAD)
+    {
+      const Piece<T>* Gb =
+          reinterpret_cast<const Piece<T>*>(G + cell0 * Q3 * 6);
+      const int npieces = ncells_blk > 0 ? (int)(3 * Q3 * ncells_blk) : 0;
+#pragma unroll
+      for (int k = 0; k < GPT; ++k)
+      {
+        const int p = k * BT + tid;
+        if (p < npieces)
+          gp[k] = stream_load(Gb + p);
+      }
+    }
+    T dw[MASSTERM ? Q : 1];
+    if constexpr (MASSTERM)
+    {
+      if (live)
+      {
+        const T* Dp = detJw + cell * Q3 + ln;
+#pragma unroll
+        for (int qa = 0; qa < Q; ++qa)
+          dw[qa] = stream_load(Dp + qa * Q2);
+      }
+    }
+
+    // ---------------- F1: contract axis b ---------------------------------
+    if (live && role1)
+    {
+      T ap[Q], ag[Q];
+#pragma unroll
+      for (int b = 0; b < N; ++b)
+      {
+        T rp[Q], rg[Q];
+        load_row<Q>(ph, b * Q, rp);
+        if constexpr (GRAD)
+          load_row<Q>(gr, b * Q, rg);
+#pragma unroll
+        for (int q = 0; q < Q; ++q)
+        {
+          ap[q] = b == 0 ? rp[q] * u[b] : fma(rp[q], u[b], ap[q]);
+          if constexpr (GRAD)
+            ag[q] = b == 0 ? rg[q] * u[b] : fma(rg[q], u[b], ag[q]);
         }
       }
-      const int o = (qa * Q + qb) * N + c;
-      Yc[Y_PP * SY + o] = spp;
-      if constexpr (GRAD)
+#pragma unroll
+      for (int q = 0; q < Q; ++q)
       {
-        Yc[Y_PG * SY + o] = spg;
-        Yc[Y_GP * SY + o] = sgp;
+        Xc[X_P * SX + q * XQ + r1c * XC + r1a] = ap[q];
+        if constexpr (GRAD)
+          Xc[X_G * SX + q * XQ + r1c * XC + r1a] = ag[q];
       }
     }
-  }
-  __syncthreads();
+    __syncthreads();
 
-  // ---------------- F3 + quadrature-point work + B3 -----------------------
-  T bpp[N], bpg[N], bgp[N];
-  if (live)
-  {
-#pragma unroll
-    for (int c = 0; c < N; ++c)
+    // ---------------- F2: contract axis c ---------------------------------
+    if (live && role2)
     {
-      bpp[c] = Yc[Y_PP * SY + ln * N + c];
-      if constexpr (GRAD)
-      {
-        bpg[c] = Yc[Y_PG * SY + ln * N + c];
-        bgp[c] = Yc[Y_GP * SY + ln * N + c];
-      }
-    }
-  }
-  __syncthreads(); // Ys is rewritten by B3 below
-
-  if (live)
-  {
-    T va[Q], vb[Q], v0[Q], vm[Q];
-#pragma unroll
-    for (int qc = 0; qc < Q; ++qc)
-    {
-      T uq = 0, dc = 0, db = 0, da = 0;
+      T inp[N], ing[N];
 #pragma unroll
       for (int c = 0; c < N; ++c)
       {
-        if constexpr (UQ)
-          uq = fma(ph[c * Q + qc], bpp[c], uq);
-        if constexpr (DC)
-          dc = fma(gr[c * Q + qc], bpp[c], dc);
+        inp[c] = Xc[X_P * SX + r2q * XQ + c * XC + r2a];
         if constexpr (GRAD)
+          ing[c] = Xc[X_G * SX + r2q * XQ + c * XC + r2a];
+      }
+      T bpp[Q], bpg[Q], bgp[Q];
+#pragma unroll
+      for (int c = 0; c < N; ++c)
+      {
+        T rp[Q], rg[Q];
+        load_row<Q>(ph, c * Q, rp);
+        if constexpr (GRAD)
+          load_row<Q>(gr, c * Q, rg);
+#pragma unroll
+        for (int q = 0; q < Q; ++q)
         {
-          db = fma(ph[c * Q + qc], bpg[c], db);
-          da = fma(ph[c * Q + qc], bgp[c], da);
+          bpp[q] = c == 0 ? rp[q] * inp[c] : fma(rp[q], inp[c], bpp[q]);
+          if constexpr (GRAD)
+          {
+            bpg[q] = c == 0 ? rg[q] * inp[c] : fma(rg[q], inp[c], bpg[q]);
+            bgp[q] = c == 0 ? rp[q] * ing[c] : fma(rp[q], ing[c], bgp[q]);
+          }
         }
       }
-      if constexpr (DC)
+      const int o = r2a * YA + r2q * YQ;
+#pragma unroll
+      for (int q = 0; q < Q; ++q)
       {
-        const T* g = g6 + 6 * qc;
-        // S = [[G5,G4,G2],[G4,G3,G1],[G2,G1,G0]] acting on (d_a, d_b, d_c)
-        va[qc] = g[5] * da + g[4] * db + g[2] * dc;
-        vb[qc] = g[4] * da + g[3] * db + g[1] * dc;
-        v0[qc] = g[2] * da + g[1] * db + g[0] * dc;
+        Yc[Y_PP * SY + o + q] = bpp[q];
+        if constexpr (GRAD)
+        {
+          Yc[Y_PG * SY + o + q] = bpg[q];
+          Yc[Y_GP * SY + o + q] = bgp[q];
+        }
       }
-      else if constexpr (LIT)
-      {
-        const T* g = g6 + 6 * qc;
-        // third row of the transform applied to (d_a, d_b, u)
-        // (src/stiffness.hpp:27 with Wz undifferentiated)
-        v0[qc] = g[2] * da + g[1] * db + g[0] * uq;
-      }
-      if constexpr (MASSTERM)
-        vm[qc] = dw[qc] * uq;
     }
-    // B3: contract axis c (transposed)
-#pragma unroll
-    for (int c = 0; c < N; ++c)
+    __syncthreads();
+
+    // ---------------- F3 + quadrature-point work + B3 ---------------------
+    T ipp[N], ipg[N], igp[N];
+    if (live)
     {
-      T ra = 0, rb = 0, r0 = 0;
 #pragma unroll
-      for (int qc = 0; qc < Q; ++qc)
+      for (int a = 0; a < N; ++a)
       {
-        if constexpr (CHAIN_AB)
-        {
-          ra = fma(bph(c, qc), va[qc], ra);
-          rb = fma(bph(c, qc), vb[qc], rb);
-        }
+        ipp[a] = Yc[Y_PP * SY + a * YA + yo3];
         if constexpr (GRAD)
-          r0 = fma(bgr(c, qc), v0[qc], r0);
+        {
+          ipg[a] = Yc[Y_PG * SY + a * YA + yo3];
+          igp[a] = Yc[Y_GP * SY + a * YA + yo3];
+        }
+      }
+    }
+    __syncthreads(); // X and Y are idle from here until B3 writes Y
+
+    if constexpr (GRAD)
+    {
+      // geometry block -> LDS, same layout as in memory ([cell][q][6])
+#pragma unroll
+      for (int k = 0; k < GPT; ++k)
+      {
+        const int p = k * BT + tid;
+        if (p < GP)
+          Ls[p] = gp[k];
+      }
+      __syncthreads();
+    }
+
+    T ra_[N], rb_[N], rc_[N]; // B3 outputs, stored after the geometry is read
+    if (live)
+    {
+      T uq[Q], da[Q], db[Q], dc[Q];
+#pragma unroll
+      for (int a = 0; a < N; ++a)
+      {
+        T rp[Q], rg[Q];
+        load_row<Q>(ph, a * Q, rp);
+        if constexpr (GRAD)
+          load_row<Q>(gr, a * Q, rg);
+#pragma unroll
+        for (int q = 0; q < Q; ++q)
+        {
+          if constexpr (UQ)
+            uq[q] = a == 0 ? rp[q] * ipp[a] : fma(rp[q], ipp[a], uq[q]);
+          if constexpr (GRAD)
+          {
+            da[q] = a == 0 ? rg[q] * ipp[a] : fma(rg[q], ipp[a], da[q]);
+            db[q] = a == 0 ? rp[q] * igp[a] : fma(rp[q], igp[a], db[q]);
+          }
+          if constexpr (FULL)
+            dc[q] = a == 0 ? rp[q] * ipg[a] : fma(rp[q], ipg[a], dc[q]);
+        }
+      }
+      // pointwise: v = S(q) d (+ detJw u)
+      T va[Q], vb[Q], vc[Q], vm[Q];
+#pragma unroll
+      for (int q = 0; q < Q; ++q)
+      {
+        T g[6];
+        if constexpr (GRAD)
+        {
+          const Piece<T>* Lg = Ls + (lc * Q3 + q * Q2 + ln) * 3;
+#pragma unroll
+          for (int j = 0; j < 3; ++j)
+          {
+            const Piece<T> pc = Lg[j];
+            g[2 * j] = pc.x;
+            g[2 * j + 1] = pc.y;
+          }
+        }
+        if constexpr (FULL)
+        {
+          // S = [[G5,G4,G2],[G4,G3,G1],[G2,G1,G0]] on (d_a, d_b, d_c)
+          va[q] = g[5] * da[q] + g[4] * db[q] + g[2] * dc[q];
+          vb[q] = g[4] * da[q] + g[3] * db[q] + g[1] * dc[q];
+          vc[q] = g[2] * da[q] + g[1] * db[q] + g[0] * dc[q];
+        }
+        else if constexpr (LIT)
+        {
+          // third row of the transform on (d_a, d_b, undifferentiated u)
+          // (src/stiffness.hpp:27 with Wz from src/stiffness.hpp:76-77)
+          vc[q] = g[2] * da[q] + g[1] * db[q] + g[0] * uq[q];
+        }
         if constexpr (MASSTERM)
-          r0 = fma(bph(c, qc), vm[qc], r0);
+          vm[q] = dw[q] * uq[q];
       }
-      const int o = ln * N + c;
-      if constexpr (CHAIN_AB)
-      {
-        Yc[Y_RA * SY + o] = ra;
-        Yc[Y_RB * SY + o] = rb;
-      }
-      Yc[Y_R0 * SY + o] = r0;
-    }
-  }
-  __syncthreads();
-
-  // ---------------- B2: contract axis b (transposed) ------------------------
-  if (live && ln < Q * N)
-  {
-    const int qa = ln / N, c = ln - (ln / N) * N;
-    T ra[Q], rb[Q], r0[Q];
+      // B3: contract axis a (transposed), one output row a at a time
 #pragma unroll
-    for (int qb = 0; qb < Q; ++qb)
-    {
-      const int o = (qa * Q + qb) * N + c;
-      if constexpr (CHAIN_AB)
+      for (int a = 0; a < N; ++a)
       {
-        ra[qb] = Yc[Y_RA * SY + o];
-        rb[qb] = Yc[Y_RB * SY + o];
-      }
-      r0[qb] = Yc[Y_R0 * SY + o];
-    }
+        T rp[Q], rg[Q];
+        load_bwd_row<T, N, Q, REF>(ph, a, rp);
+        if constexpr (FULL)
+          load_bwd_row<T, N, Q, REF>(gr, a, rg);
+        T ra = 0, rb = 0, rc = 0;
 #pragma unroll
-    for (int b = 0; b < N; ++b)
-    {
-      T sa = 0, s0 = 0;
-#pragma unroll
-      for (int qb = 0; qb < Q; ++qb)
-      {
-        if constexpr (CHAIN_AB)
+        for (int q = 0; q < Q; ++q)
         {
-          sa = fma(bph(b, qb), ra[qb], sa);
-          s0 = fma(bgr(b, qb), rb[qb], s0);
+          if constexpr (FULL)
+          {
+            ra = fma(rg[q], va[q], ra);
+            rb = fma(rp[q], vb[q], rb);
+          }
+          if constexpr (GRAD)
+            rc = fma(rp[q], vc[q], rc);
+          if constexpr (MASSTERM)
+            ra = fma(rp[q], vm[q], ra);
         }
-        s0 = fma(bph(b, qb), r0[qb], s0);
+        ra_[a] = ra;
+        rb_[a] = rb;
+        rc_[a] = rc;
       }
-      const int o = qa * NN + b * N + c;
-      if constexpr (CHAIN_AB)
-        Xc[X_SA * SX + o] = sa;
-      Xc[X_S0 * SX + o] = s0;
     }
-  }
-  __syncthreads();
-
-  // ---------------- B1: contract axis a (transposed) + store/scatter ------
-  if (live && ln < NN)
-  {
-    T sa[Q], s0[Q];
-#pragma unroll
-    for (int qa = 0; qa < Q; ++qa)
+    if constexpr (GRAD)
+      __syncthreads(); // the geometry in LDS is consumed; Y may be rewritten
+    if (live)
     {
-      if constexpr (CHAIN_AB)
-        sa[qa] = Xc[X_SA * SX + qa * NN + ln];
-      s0[qa] = Xc[X_S0 * SX + qa * NN + ln];
-    }
-    const long long base = cell * N3 + ln;
 #pragma unroll
-    for (int a = 0; a < N; ++a)
-    {
-      T w = 0;
-#pragma unroll
-      for (int qa = 0; qa < Q; ++qa)
+      for (int a = 0; a < N; ++a)
       {
-        if constexpr (CHAIN_AB)
-          w = fma(bgr(a, qa), sa[qa], w);
-        w = fma(bph(a, qa), s0[qa], w);
+        const int o = a * YA + yo3;
+        if constexpr (FULL || MASSTERM)
+          Yc[Y_RA * SY + o] = ra_[a];
+        if constexpr (FULL)
+          Yc[Y_RB * SY + o] = rb_[a];
+        if constexpr (GRAD)
+          Yc[Y_RC * SY + o] = rc_[a];
       }
-      if constexpr (GATHER)
-        atomic_add(y + idx[a], w);
-      else
-        y[base + a * NN] = w;
     }
+    __syncthreads();
+
+    // ---------------- B2: contract axis c (transposed) --------------------
+    if (live && role2)
+    {
+      T ra[Q], rb[Q], rc[Q];
+      const int o = r2a * YA + r2q * YQ;
+#pragma unroll
+      for (int q = 0; q < Q; ++q)
+      {
+        if constexpr (FULL || MASSTERM)
+          ra[q] = Yc[Y_RA * SY + o + q];
+        if constexpr (FULL)
+          rb[q] = Yc[Y_RB * SY + o + q];
+        if constexpr (GRAD)
+          rc[q] = Yc[Y_RC * SY + o + q];
+      }
+#pragma unroll
+      for (int c = 0; c < N; ++c)
+      {
+        T rp[Q], rg[Q];
+        load_bwd_row<T, N, Q, REF>(ph, c, rp);
+        if constexpr (GRAD)
+          load_bwd_row<T, N, Q, REF>(gr, c, rg);
+        T s0 = 0, sb = 0;
+#pragma unroll
+        for (int q = 0; q < Q; ++q)
+        {
+          if constexpr (FULL || MASSTERM)
+            s0 = fma(rp[q], ra[q], s0);
+          if constexpr (GRAD)
+            s0 = fma(rg[q], rc[q], s0);
+          if constexpr (FULL)
+            sb = fma(rp[q], rb[q], sb);
+        }
+        const int xo = r2q * XQ + c * XC + r2a;
+        Xc[X_S0 * SX + xo] = s0;
+        if constexpr (FULL)
+          Xc[X_SB * SX + xo] = sb;
+      }
+    }
+    __syncthreads();
+
+    // ---------------- B1: contract axis b (transposed) + store/scatter ----
+    if (live && role1)
+    {
+      T s0[Q], sb[Q];
+#pragma unroll
+      for (int q = 0; q < Q; ++q)
+      {
+        s0[q] = Xc[X_S0 * SX + q * XQ + r1c * XC + r1a];
+        if constexpr (FULL)
+          sb[q] = Xc[X_SB * SX + q * XQ + r1c * XC + r1a];
+      }
+#pragma unroll
+      for (int b = 0; b < N; ++b)
+      {
+        T rp[Q], rg[Q];
+        load_bwd_row<T, N, Q, REF>(ph, b, rp);
+        if constexpr (FULL)
+          load_bwd_row<T, N, Q, REF>(gr, b, rg);
+        T acc = 0;
+#pragma unroll
+        for (int q = 0; q < Q; ++q)
+        {
+          acc = fma(rp[q], s0[q], acc);
+          if constexpr (FULL)
+            acc = fma(rg[q], sb[q], acc);
+        }
+        if constexpr (GATHER)
+          atomic_add(y + idx[b], acc);
+        else
+          y[cell * N3 + uoff + b * N] = acc;
+      }
+    }
+
   }
 }
 
-// Launch one instance; the grid covers num_cells in blocks of CPB cells.
+// Launch one instance: one workgroup per batch of CPB cells.
 template <typename T, int N, int Q, int OP, bool REF, bool GATHER>
 int launch_cells(const ApplyReq& r)
 {
   constexpr int CPB = cells_per_block<Q>();
-  const long long nblocks = (r.num_cells + CPB - 1) / CPB;
-  if (nblocks == 0)
+  const long long nblk = (r.num_cells + CPB - 1) / CPB;
+  if (nblk == 0)
     return FEMTC_OK;
-  if (nblocks > 0x7fffffffLL)
+  if (nblk > 0x7fffffffLL)
     return FEMTC_ERR_INVALID_ARGUMENT;
   hipLaunchKernelGGL((apply_cells<T, N, Q, OP, REF, GATHER>),
-                     dim3((unsigned)nblocks), dim3(block_threads<Q>()), 0,
+                     dim3((unsigned)nblk), dim3(block_threads<Q>()), 0,
                      r.stream, (const T*)r.tab, (const T*)r.x, (T*)r.y,
                      (const T*)r.G, (const T*)r.detJw, r.dofmap,
                      (long long)r.num_cells);

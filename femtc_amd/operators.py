@@ -129,6 +129,12 @@ def kpm_apply(tables, U, W, G, detJw, num_cells: int, degree: int, nq: int,
         _ptr(detJw), num_cells, _stream(stream)), "kpm_apply")
 
 
+def fill_zero(y, stream=None) -> None:
+    """y = 0 with a device kernel."""
+    _lib.check(_lib.lib().femtc_fill_zero(_ptr(y), y.numel(), _dtype_code(y),
+                                          _stream(stream)), "fill_zero")
+
+
 def global_apply(op: int, tables, x, y, dofmap, num_cells: int, degree: int,
                  nq: int, G=None, detJw=None, stream=None,
                  accumulate: bool = False) -> None:

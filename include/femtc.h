@@ -158,6 +158,9 @@ int femtc_global_apply_add(int op, int dtype, int degree, int nq,
                            int64_t ndofs, const int32_t* dofmap, const void* G,
                            const void* detJw, int64_t num_cells, void* stream);
 
+/* y[0..n) = 0 with a kernel (usable under graph capture). */
+int femtc_fill_zero(void* y, int64_t n, int dtype, void* stream);
+
 /* ---------------------------------------------------------------------- */
 /* Reference-shaped synchronous entry points on HOST buffers: the C side   */
 /* of the operators::mass_operator<T> / stiffness_operator<T> shims        */

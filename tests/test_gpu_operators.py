@@ -222,3 +222,79 @@ def test_config2_stiffness_global_full_size():
     F.global_apply(F.OP_STIFFNESS, dev(tab), dev(x), dy, dev(dm, torch.int32),
                    C, P, Q, G=dev(G))
     assert rel(host(dy), ref) <= TOL64
+
+
+def _box(nx, ny, nz, P, Q, seed):
+    C = nx * ny * nz
+    J = O.random_jacobians(seed, C)
+    G, Dw = O.geometry_affine(J, Q)
+    dm = O.dofmap_structured(nx, ny, nz, P)
+    ng = (nx * P + 1) * (ny * P + 1) * (nz * P + 1)
+    return C, G, Dw, dm, ng
+
+
+BOXES = [(2, 3, 8, 4, 5), (3, 2, 4, 6, 7), (2, 2, 8, 3, 5), (2, 2, 5, 2, 3),
+         (2, 2, 4, 7, 8), (2, 1, 4, 8, 9), (1, 1, 32, 1, 2)]
+
+
+@pytest.mark.parametrize("nx,ny,nz,P,Q", BOXES)
+@pytest.mark.parametrize("op", [F.OP_MASS, F.OP_STIFFNESS, F.OP_KPM])
+def test_global_overwrite_and_accumulate(op, nx, ny, nz, P, Q):
+    C, G, Dw, dm, ng = _box(nx, ny, nz, P, Q, 40 + P)
+    tab = O.basis_tables(P, Q)
+    x = O.uniform(41, ng)
+    ref = O.global_operator(op, P + 1, Q, tab, x, dm, G, Dw)
+    dtab, dx, ddm = dev(tab), dev(x), dev(dm, torch.int32)
+    dG, dD = dev(G), dev(Dw)
+    # overwrite: prior content of y is ignored
+    dy = torch.full((ng,), 5.0, device="cuda", dtype=torch.float64)
+    F.global_apply(op, dtab, dx, dy, ddm, C, P, Q, G=dG, detJw=dD)
+    assert rel(host(dy), ref) <= TOL64
+    # accumulate onto a non-zero y
+    y0 = O.uniform(42, ng)
+    dy = dev(y0)
+    F.global_apply(op, dtab, dx, dy, ddm, C, P, Q, G=dG, detJw=dD,
+                   accumulate=True)
+    assert rel(host(dy), ref + y0) <= TOL64
+    # the library's fill kernel then accumulate
+    F.fill_zero(dy)
+    F.global_apply(op, dtab, dx, dy, ddm, C, P, Q, G=dG, detJw=dD,
+                   accumulate=True)
+    assert rel(host(dy), ref) <= TOL64
+
+
+@pytest.mark.parametrize("nx,ny,nz,P,Q", [(2, 3, 8, 4, 5), (3, 2, 4, 6, 7)])
+def test_global_kpm_f32(nx, ny, nz, P, Q):
+    C, G, Dw, dm, ng = _box(nx, ny, nz, P, Q, 7)
+    tab = O.basis_tables(P, Q)
+    x = O.uniform(8, ng)
+    ref = O.global_operator(F.OP_KPM, P + 1, Q, tab, x, dm, G, Dw)
+    f = torch.float32
+    dy = torch.empty(ng, device="cuda", dtype=f)
+    F.global_apply(F.OP_KPM, dev(tab, f), dev(x, f), dy, dev(dm, torch.int32),
+                   C, P, Q, G=dev(G, f), detJw=dev(Dw, f))
+    assert rel(host(dy), ref) <= TOL32
+
+
+def test_config3_stiffness_global_full_size():
+    # 48^3 mesh, P=6 GLL, Q=7 Gauss: the benchmark workload
+    n, P, Q = 48, 6, 7
+    C, G, Dw, dm, ng = _box(n, n, n, P, Q, 3)
+    tab = O.basis_tables(P, Q)
+    x = O.uniform(4, ng)
+    ref = O.global_operator(F.OP_STIFFNESS, P + 1, Q, tab, x, dm, G, None)
+    dy = torch.empty(ng, device="cuda", dtype=torch.float64)
+    F.global_apply(F.OP_STIFFNESS, dev(tab), dev(x), dy, dev(dm, torch.int32),
+                   C, P, Q, G=dev(G))
+    assert rel(host(dy), ref) <= TOL64
+
+
+def test_misaligned_geometry_is_rejected():
+    P, Q, C = 2, 3, 4
+    N = P + 1
+    buf = torch.zeros(C * Q ** 3 * 6 + 1, device="cuda", dtype=torch.float64)
+    U = torch.zeros(C * N ** 3, device="cuda", dtype=torch.float64)
+    W = torch.zeros_like(U)
+    tab = dev(O.basis_tables(P, Q))
+    with pytest.raises(F.FemtcError, match="aligned"):
+        F.stiffness_apply(tab, U, W, buf[1:].data_ptr(), C, P, Q)
