@@ -43,7 +43,11 @@
 //
 // LDS layout.  X[qb][c][a] (F1 -> F2, B2 -> B1) and Y[a][qb][qc] (F2 -> F3,
 // B3 -> B2) with the padded strides of lds_layout.hpp, which keep both lane
-// patterns of each tensor (nearly) free of bank conflicts.
+// patterns of each tensor (nearly) free of bank conflicts.  X and Y are never
+// live together, so they share one area (a barrier separates the last read of
+// one from the first write of the other), and the geometry passes through the
+// same area in two halves of qa-planes at the quadrature stage.  This keeps a
+// workgroup's LDS small enough that registers, not LDS, bound occupancy.
 //
 // Table access.  Every contraction walks the table one dof row at a time
 // (forward: acc[q] += T[d][q] in[d]; backward: out[d] = sum_q T[d][q] in[q]),
@@ -119,7 +123,40 @@ __device__ inline void load_bwd_row(const T* __restrict__ tab, int d, T (&row)[Q
     load_row<Q>(tab, d * Q, row);
 }
 
-// LDS elements of the work area (X, Y and the geometry alias) of one block.
+// The geometry block is staged through LDS in two halves of qa-planes:
+// planes [0, QA1) and [QA1, Q).
+template <int Q>
+constexpr int geometry_split()
+{
+  return (Q + 1) / 2;
+}
+
+// Lane-contiguous pieces of the qa-planes [QB0, QB0+QH) of the geometry of
+// the block's cells (cell0 .. cell0+ncells_blk-1), as they are laid out in
+// LDS: [cell][plane][qb][qc][6].
+template <typename T, int Q, int BT, int QB0, int QH, int GPTH>
+__device__ inline void load_geometry_half(const T* __restrict__ G,
+                                          long long cell0,
+                                          long long ncells_blk, int tid,
+                                          Piece<T> (&gp)[GPTH])
+{
+  constexpr int Q3 = Q * Q * Q;
+  constexpr int PER_CELL = 3 * QH * Q * Q; // pieces of one cell's half
+  const Piece<T>* Gb = reinterpret_cast<const Piece<T>*>(G + cell0 * Q3 * 6) +
+                       QB0 * 3 * Q * Q;
+#pragma unroll
+  for (int k = 0; k < GPTH; ++k)
+  {
+    const int p = k * BT + tid;
+    const int c = p / PER_CELL;
+    if (c < ncells_blk)
+      gp[k] = stream_load(Gb + c * 3 * Q3 + (p - c * PER_CELL));
+  }
+}
+
+// LDS elements of one block: the work tensors X and Y share one area (they
+// are never live together), and a half of the geometry block aliases it at
+// the quadrature stage.
 template <typename T, int N, int Q, int OP>
 constexpr int work_lds_elems()
 {
@@ -128,8 +165,9 @@ constexpr int work_lds_elems()
   constexpr LdsLayout LY = lds_layout<(int)sizeof(T), N, Q>();
   constexpr int XCELLS = CPB > 1 ? LY.XCELL : (GRAD ? 2 : 1) * Q * LY.XQ;
   constexpr int YCELLS = CPB > 1 ? LY.YCELL : (GRAD ? 3 : 1) * N * LY.YA;
-  constexpr int WORK = CPB * (XCELLS + YCELLS);
-  constexpr int GELEM = GRAD ? 6 * Q * Q * Q * CPB : 0;
+  constexpr int WCELL = XCELLS > YCELLS ? XCELLS : YCELLS;
+  constexpr int WORK = CPB * WCELL;
+  constexpr int GELEM = GRAD ? 6 * geometry_split<Q>() * Q * Q * CPB : 0;
   return WORK > GELEM ? WORK : GELEM;
 }
 
@@ -162,11 +200,16 @@ __global__ void __launch_bounds__(block_threads<Q>())
   constexpr int N3 = N * N * N;
   constexpr int Q2 = Q * Q;
   constexpr int Q3 = Q * Q * Q;
-  // geometry block of the workgroup, in pieces of two elements
-  constexpr int GP = GRAD ? 3 * Q3 * CPB : 0;
-  constexpr int GPT = (GP + BT - 1) / BT; // pieces per thread
-  // LDS: X then Y per cell; the geometry block aliases them at the
-  // quadrature stage, when both are idle.
+  // geometry of the workgroup in pieces of two elements, in two halves of
+  // qa-planes: [0, QA1) and [QA1, Q); GP0/GP1 pieces, GPT0/GPT1 per thread
+  constexpr int QA1 = geometry_split<Q>();
+  constexpr int GP0 = GRAD ? 3 * QA1 * Q2 * CPB : 0;
+  constexpr int GP1 = GRAD ? 3 * (Q - QA1) * Q2 * CPB : 0;
+  constexpr int GPT0 = (GP0 + BT - 1) / BT;
+  constexpr int GPT1 = (GP1 + BT - 1) / BT;
+  // LDS: X and Y share one area per cell (WCELL); a geometry half aliases
+  // the whole area at the quadrature stage, when neither is live.
+  constexpr int WCELL = XCELLS > YCELLS ? XCELLS : YCELLS;
   constexpr int LDSW = (work_lds_elems<T, N, Q, OP>() + 1) / 2; // pieces
   // LDS slots
   constexpr int X_P = 0, X_G = 1;              // F1 -> F2
@@ -184,8 +227,8 @@ __global__ void __launch_bounds__(block_threads<Q>())
   const int lc = tid / NL;
   const int ln = tid - lc * NL;
   const bool lane_ok = lc < CPB;
-  T* Xc = L + (lane_ok ? lc : 0) * XCELLS;
-  T* Yc = L + CPB * XCELLS + (lane_ok ? lc : 0) * YCELLS;
+  T* Xc = L + (lane_ok ? lc : 0) * WCELL;
+  T* Yc = Xc;
 
   // thread roles
   const int r1a = ln / N, r1c = ln - (ln / N) * N;  // F1/B1: (a, c)
@@ -219,19 +262,12 @@ __global__ void __launch_bounds__(block_threads<Q>())
           u[b] = x[cell * N3 + uoff + b * N];
       }
     }
-    Piece<T> gp[GRAD ? GPT : 1];
+    Piece<T> gp0[GRAD ? GPT0 : 1], gp1[GRAD ? GPT1 : 1];
     if constexpr (GRAD)
     {
-      const Piece<T>* Gb =
-          reinterpret_cast<const Piece<T>*>(G + cell0 * Q3 * 6);
-      const int npieces = ncells_blk > 0 ? (int)(3 * Q3 * ncells_blk) : 0;
-#pragma unroll
-      for (int k = 0; k < GPT; ++k)
-      {
-        const int p = k * BT + tid;
-        if (p < npieces)
-          gp[k] = stream_load(Gb + p);
-      }
+      load_geometry_half<T, Q, BT, 0, QA1>(G, cell0, ncells_blk, tid, gp0);
+      load_geometry_half<T, Q, BT, QA1, Q - QA1>(G, cell0, ncells_blk, tid,
+                                                  gp1);
     }
     T dw[MASSTERM ? Q : 1];
     if constexpr (MASSTERM)
@@ -275,6 +311,7 @@ __global__ void __launch_bounds__(block_threads<Q>())
     __syncthreads();
 
     // ---------------- F2: contract axis c ---------------------------------
+    T bpp[Q], bpg[Q], bgp[Q];
     if (live && role2)
     {
       T inp[N], ing[N];
@@ -285,7 +322,6 @@ __global__ void __launch_bounds__(block_threads<Q>())
         if constexpr (GRAD)
           ing[c] = Xc[X_G * SX + r2q * XQ + c * XC + r2a];
       }
-      T bpp[Q], bpg[Q], bgp[Q];
 #pragma unroll
       for (int c = 0; c < N; ++c)
       {
@@ -304,6 +340,10 @@ __global__ void __launch_bounds__(block_threads<Q>())
           }
         }
       }
+    }
+    __syncthreads(); // X is read; Y overwrites the same LDS
+    if (live && role2)
+    {
       const int o = r2a * YA + r2q * YQ;
 #pragma unroll
       for (int q = 0; q < Q; ++q)
@@ -334,24 +374,23 @@ __global__ void __launch_bounds__(block_threads<Q>())
       }
     }
     __syncthreads(); // X and Y are idle from here until B3 writes Y
-
+    // first geometry half into LDS (its registers are freed before F3)
     if constexpr (GRAD)
     {
-      // geometry block -> LDS, same layout as in memory ([cell][q][6])
 #pragma unroll
-      for (int k = 0; k < GPT; ++k)
+      for (int k = 0; k < GPT0; ++k)
       {
         const int p = k * BT + tid;
-        if (p < GP)
-          Ls[p] = gp[k];
+        if (p < GP0)
+          Ls[p] = gp0[k];
       }
       __syncthreads();
     }
 
-    T ra_[N], rb_[N], rc_[N]; // B3 outputs, stored after the geometry is read
+    // F3: contract axis a
+    T uq[Q], da[Q], db[Q], dc[Q];
     if (live)
     {
-      T uq[Q], da[Q], db[Q], dc[Q];
 #pragma unroll
       for (int a = 0; a < N; ++a)
       {
@@ -373,39 +412,65 @@ __global__ void __launch_bounds__(block_threads<Q>())
             dc[q] = a == 0 ? rp[q] * ipg[a] : fma(rp[q], ipg[a], dc[q]);
         }
       }
-      // pointwise: v = S(q) d (+ detJw u)
-      T va[Q], vb[Q], vc[Q], vm[Q];
-#pragma unroll
-      for (int q = 0; q < Q; ++q)
+    }
+
+    // pointwise: v = S(q) d (+ detJw u), the geometry staged through LDS one
+    // half of the qa-planes at a time
+    T va[Q], vb[Q], vc[Q], vm[Q];
+    auto pointwise = [&](int q, int qa0, int per_cell) {
+      T g[6];
+      if constexpr (GRAD)
       {
-        T g[6];
-        if constexpr (GRAD)
-        {
-          const Piece<T>* Lg = Ls + (lc * Q3 + q * Q2 + ln) * 3;
+        const Piece<T>* Lg =
+            Ls + lc * per_cell + ((q - qa0) * Q2 + ln) * 3;
 #pragma unroll
-          for (int j = 0; j < 3; ++j)
-          {
-            const Piece<T> pc = Lg[j];
-            g[2 * j] = pc.x;
-            g[2 * j + 1] = pc.y;
-          }
-        }
-        if constexpr (FULL)
+        for (int j = 0; j < 3; ++j)
         {
-          // S = [[G5,G4,G2],[G4,G3,G1],[G2,G1,G0]] on (d_a, d_b, d_c)
-          va[q] = g[5] * da[q] + g[4] * db[q] + g[2] * dc[q];
-          vb[q] = g[4] * da[q] + g[3] * db[q] + g[1] * dc[q];
-          vc[q] = g[2] * da[q] + g[1] * db[q] + g[0] * dc[q];
+          const Piece<T> pc = Lg[j];
+          g[2 * j] = pc.x;
+          g[2 * j + 1] = pc.y;
         }
-        else if constexpr (LIT)
-        {
-          // third row of the transform on (d_a, d_b, undifferentiated u)
-          // (src/stiffness.hpp:27 with Wz from src/stiffness.hpp:76-77)
-          vc[q] = g[2] * da[q] + g[1] * db[q] + g[0] * uq[q];
-        }
-        if constexpr (MASSTERM)
-          vm[q] = dw[q] * uq[q];
       }
+      if constexpr (FULL)
+      {
+        // S = [[G5,G4,G2],[G4,G3,G1],[G2,G1,G0]] on (d_a, d_b, d_c)
+        va[q] = g[5] * da[q] + g[4] * db[q] + g[2] * dc[q];
+        vb[q] = g[4] * da[q] + g[3] * db[q] + g[1] * dc[q];
+        vc[q] = g[2] * da[q] + g[1] * db[q] + g[0] * dc[q];
+      }
+      else if constexpr (LIT)
+      {
+        // third row of the transform on (d_a, d_b, undifferentiated u)
+        // (src/stiffness.hpp:27 with Wz from src/stiffness.hpp:76-77)
+        vc[q] = g[2] * da[q] + g[1] * db[q] + g[0] * uq[q];
+      }
+      if constexpr (MASSTERM)
+        vm[q] = dw[q] * uq[q];
+    };
+    if (live)
+    {
+#pragma unroll
+      for (int q = 0; q < QA1; ++q)
+        pointwise(q, 0, 3 * QA1 * Q2);
+    }
+    if constexpr (GRAD)
+    {
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < GPT1; ++k)
+      {
+        const int p = k * BT + tid;
+        if (p < GP1)
+          Ls[p] = gp1[k];
+      }
+      __syncthreads();
+    }
+    T ra_[N], rb_[N], rc_[N]; // B3 outputs, stored after the geometry is read
+    if (live)
+    {
+#pragma unroll
+      for (int q = QA1; q < Q; ++q)
+        pointwise(q, QA1, 3 * (Q - QA1) * Q2);
       // B3: contract axis a (transposed), one output row a at a time
 #pragma unroll
       for (int a = 0; a < N; ++a)
@@ -452,9 +517,9 @@ __global__ void __launch_bounds__(block_threads<Q>())
     __syncthreads();
 
     // ---------------- B2: contract axis c (transposed) --------------------
+    T ra[Q], rb[Q], rc[Q];
     if (live && role2)
     {
-      T ra[Q], rb[Q], rc[Q];
       const int o = r2a * YA + r2q * YQ;
 #pragma unroll
       for (int q = 0; q < Q; ++q)
@@ -466,6 +531,10 @@ __global__ void __launch_bounds__(block_threads<Q>())
         if constexpr (GRAD)
           rc[q] = Yc[Y_RC * SY + o + q];
       }
+    }
+    __syncthreads(); // Y is read; X overwrites the same LDS
+    if (live && role2)
+    {
 #pragma unroll
       for (int c = 0; c < N; ++c)
       {
