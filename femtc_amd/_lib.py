@@ -66,7 +66,28 @@ SIGNATURES = {
                                         _i32, _i32, _i32, _i32, _i32]),
     "femtc_dofmap_structured": (_i32, [_i32, _i32, _i32, _i32, _vp, _vp]),
     "femtc_geometry_affine": (_i32, [_i64, _i32, _vp, _vp, _vp, _i32, _vp]),
+    "femtc_global_diagonal": (_i32, [_i32, _i32, _i32, _i32, _vp, _vp, _i64,
+                                     _vp, _vp, _vp, _i64, _vp]),
+    "femtc_solver_create": (_i32, [C.POINTER(_vp), _vp, _i32, _vp]),
+    "femtc_solver_solve": (_i32, [_vp, _vp, _vp, C.c_double, C.c_double, _i32,
+                                  _vp]),
+    "femtc_solver_diagonal": (_i32, [_vp, C.POINTER(_vp)]),
+    "femtc_solver_destroy": (_i32, [_vp]),
 }
+
+
+class Problem(C.Structure):
+    """femtc_problem (include/femtc.h)."""
+    _fields_ = [("op", _i32), ("degree", _i32), ("nq", _i32),
+                ("num_cells", _i64), ("ndofs", _i64), ("dofmap", _vp),
+                ("tables", _vp), ("G", _vp), ("detJw", _vp)]
+
+
+class SolveInfo(C.Structure):
+    """femtc_solve_info (include/femtc.h)."""
+    _fields_ = [("status", _i32), ("outer_iterations", _i32),
+                ("inner_iterations", _i32), ("relres", C.c_double),
+                ("seconds", C.c_double)]
 
 
 class FemtcError(RuntimeError):

@@ -8,8 +8,6 @@
 
 namespace femtc
 {
-int fill_zero(void* y, int64_t n, int dtype, hipStream_t s);
-
 namespace
 {
 thread_local std::string g_last_error;
@@ -29,6 +27,7 @@ ApplyLauncher launcher_for(int N)
   }
   return nullptr;
 }
+} // namespace
 
 int fail(int status, const std::string& msg)
 {
@@ -83,6 +82,8 @@ int run(const ApplyReq& r)
   return st;
 }
 
+namespace
+{
 size_t dsize(int dtype) { return dtype == FEMTC_F64 ? 8 : 4; }
 
 // RAII device buffer for the synchronous host entry points.

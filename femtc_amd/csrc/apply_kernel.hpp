@@ -176,8 +176,10 @@ __global__ void __launch_bounds__(block_threads<Q>())
     apply_cells(const T* __restrict__ tab, const T* __restrict__ x,
                 T* __restrict__ y, const T* __restrict__ G,
                 const T* __restrict__ detJw, const int32_t* __restrict__ dofmap,
-                long long num_cells)
+                long long num_cells, const int* __restrict__ skip)
 {
+  if (skip && *skip)
+    return;
   constexpr int CPB = cells_per_block<Q>();
   constexpr int BT = block_threads<Q>();
   constexpr int NL = Q * Q;
@@ -611,7 +613,7 @@ int launch_cells(const ApplyReq& r)
                      dim3((unsigned)nblk), dim3(block_threads<Q>()), 0,
                      r.stream, (const T*)r.tab, (const T*)r.x, (T*)r.y,
                      (const T*)r.G, (const T*)r.detJw, r.dofmap,
-                     (long long)r.num_cells);
+                     (long long)r.num_cells, r.skip);
   return hip_status(hipGetLastError());
 }
 

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <string>
 
 #include "femtc.h"
 
@@ -35,6 +36,9 @@ struct ApplyReq
   const void* detJw = nullptr;
   const int32_t* dofmap = nullptr;
   int64_t num_cells = 0;
+  // optional device flag: when *skip != 0 at launch time the kernel does
+  // nothing (lets a solver queue iterations past convergence for free)
+  const int* skip = nullptr;
   hipStream_t stream = nullptr;
 };
 
@@ -54,5 +58,12 @@ inline int hip_status(hipError_t e)
 {
   return e == hipSuccess ? FEMTC_OK : FEMTC_ERR_HIP;
 }
+
+// Shared validation and dispatch (api.cpp).
+int fail(int status, const std::string& msg);
+int check_shape(int degree, int nq, int mode);
+int check_dtype(int dtype);
+int run(const ApplyReq& r); // validated request -> per-degree launcher
+int fill_zero(void* y, int64_t n, int dtype, hipStream_t s);
 
 } // namespace femtc

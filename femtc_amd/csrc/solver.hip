@@ -1,0 +1,970 @@
+// Device-resident Jacobi-preconditioned conjugate gradients on the continuous
+// operator A = M, K or K+M (applied through the dofmap), and FP64 defect
+// correction around a lower-precision inner PCG.
+//
+// Plain solve (inner dtype FP64): PCG on A u = f in FP64.
+// Refined solve (inner dtype FP32): the outer loop keeps u, f and the
+// residual r = f - A u in FP64 (geometry, tables and the operator in FP64);
+// each step solves A_32 e = r / ||r|| with FP32 vectors, FP32 tables and an
+// FP32 copy of the geometry to a loose tolerance, then u += ||r|| e and the
+// residual is recomputed with the FP64 operator.  Every dot product and norm
+// is accumulated in FP64, in a fixed order (per-block partial sums, then one
+// block), so a solve is deterministic up to the operator's atomic scatter.
+//
+// The CG scalars live on the device (PcgState): the host queues batches of
+// iterations without reading anything back; a reduction kernel raises
+// `done` on convergence, breakdown or the iteration cap, and every kernel of
+// the iteration (the operator included) returns at once when it is set.  The
+// host checks `done` once per batch.
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+
+#include "common.hpp"
+
+namespace femtc
+{
+namespace
+{
+
+constexpr int RB = 256;    // threads per block, vector kernels
+constexpr int NBLK = 1024; // blocks of the vector kernels (fixed: the partial
+                           // sums are always combined in the same order)
+
+struct PcgState
+{
+  double rz, rr, pAp, alpha, beta, tol2, rr0;
+  int it, maxit, done, status;
+};
+
+// Sum of NV values over the block; the result is valid in thread 0.
+template <int NV>
+__device__ inline void block_sum(double (&v)[NV])
+{
+  __shared__ double sh[NV][RB / 64];
+#pragma unroll
+  for (int k = 0; k < NV; ++k)
+  {
+    double x = v[k];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+      x += __shfl_down(x, o, 64);
+    v[k] = x;
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0)
+  {
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+      sh[k][w] = v[k];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0)
+  {
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+    {
+      double s = 0.0;
+      for (int j = 0; j < RB / 64; ++j)
+        s += sh[k][j];
+      v[k] = s;
+    }
+  }
+}
+
+template <int NV>
+__device__ inline void store_partials(double (&v)[NV], double* part)
+{
+  block_sum<NV>(v);
+  if (threadIdx.x == 0)
+  {
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+      part[k * NBLK + blockIdx.x] = v[k];
+  }
+}
+
+// Combine the NBLK partials of NV sums (one block of RB threads).
+template <int NV>
+__device__ inline void combine_partials(const double* part, double (&v)[NV])
+{
+#pragma unroll
+  for (int k = 0; k < NV; ++k)
+  {
+    double s = 0.0;
+    for (int j = threadIdx.x; j < NBLK; j += RB)
+      s += part[k * NBLK + j];
+    v[k] = s;
+  }
+  block_sum<NV>(v);
+}
+
+#define GRID_LOOP(i, n)                                                      \
+  for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < (n);          \
+       i += (int64_t)NBLK * RB)
+
+// ---------------------------------------------------------------- PCG ----
+
+// e = 0, r = rhs, p = z = D^-1 r, Ap = 0; partials of r.z and r.r
+template <typename V>
+__global__ void __launch_bounds__(RB)
+    k_start(const V* __restrict__ rhs, V* __restrict__ e, V* __restrict__ r,
+            V* __restrict__ p, V* __restrict__ Ap, const V* __restrict__ dinv,
+            int64_t n, double* __restrict__ part)
+{
+  double s[2] = {0.0, 0.0};
+  GRID_LOOP(i, n)
+  {
+    const V ri = rhs[i];
+    const V zi = dinv[i] * ri;
+    e[i] = V(0);
+    r[i] = ri;
+    p[i] = zi;
+    Ap[i] = V(0);
+    s[0] += (double)ri * (double)zi;
+    s[1] += (double)ri * (double)ri;
+  }
+  store_partials<2>(s, part);
+}
+
+__global__ void __launch_bounds__(RB)
+    k_reduce_start(const double* __restrict__ part, PcgState* st, double rtol,
+                   int maxit)
+{
+  double v[2];
+  combine_partials<2>(part, v);
+  if (threadIdx.x == 0)
+  {
+    PcgState s{};
+    s.rz = v[0];
+    s.rr = v[1];
+    s.rr0 = v[1];
+    s.tol2 = rtol * rtol * v[1];
+    s.maxit = maxit;
+    s.it = 0;
+    s.status = FEMTC_OK;
+    s.done = 0;
+    if (!(v[1] > s.tol2) || v[1] == 0.0)
+      s.done = 1; // zero right-hand side (or nothing to do)
+    if (!__builtin_isfinite(v[0]) || !__builtin_isfinite(v[1]))
+    {
+      s.done = 1;
+      s.status = FEMTC_ERR_BREAKDOWN;
+    }
+    *st = s;
+  }
+}
+
+template <typename V>
+__global__ void __launch_bounds__(RB)
+    k_dot(const V* __restrict__ p, const V* __restrict__ Ap, int64_t n,
+          double* __restrict__ part, const PcgState* __restrict__ st)
+{
+  if (st->done)
+    return;
+  double s[1] = {0.0};
+  GRID_LOOP(i, n) { s[0] += (double)p[i] * (double)Ap[i]; }
+  store_partials<1>(s, part);
+}
+
+__global__ void __launch_bounds__(RB)
+    k_alpha(const double* __restrict__ part, PcgState* st)
+{
+  if (st->done)
+    return;
+  double v[1];
+  combine_partials<1>(part, v);
+  if (threadIdx.x == 0)
+  {
+    st->pAp = v[0];
+    if (!(v[0] > 0.0) || !__builtin_isfinite(v[0]))
+    {
+      st->done = 1;
+      st->status = FEMTC_ERR_BREAKDOWN;
+    }
+    else
+      st->alpha = st->rz / v[0];
+  }
+}
+
+// e += alpha p, r -= alpha Ap, Ap = 0 (ready for the next scatter);
+// partials of r.z and r.r with z = D^-1 r
+template <typename V>
+__global__ void __launch_bounds__(RB)
+    k_update(V* __restrict__ e, const V* __restrict__ p, V* __restrict__ r,
+             V* __restrict__ Ap, const V* __restrict__ dinv, int64_t n,
+             const PcgState* __restrict__ st, double* __restrict__ part)
+{
+  if (st->done)
+    return;
+  const V a = (V)st->alpha;
+  double s[2] = {0.0, 0.0};
+  GRID_LOOP(i, n)
+  {
+    const V ri = r[i] - a * Ap[i];
+    e[i] += a * p[i];
+    r[i] = ri;
+    Ap[i] = V(0);
+    const V zi = dinv[i] * ri;
+    s[0] += (double)ri * (double)zi;
+    s[1] += (double)ri * (double)ri;
+  }
+  store_partials<2>(s, part);
+}
+
+__global__ void __launch_bounds__(RB)
+    k_beta(const double* __restrict__ part, PcgState* st)
+{
+  if (st->done)
+    return;
+  double v[2];
+  combine_partials<2>(part, v);
+  if (threadIdx.x == 0)
+  {
+    const double rzn = v[0];
+    st->beta = rzn / st->rz;
+    st->rz = rzn;
+    st->rr = v[1];
+    st->it += 1;
+    if (!__builtin_isfinite(v[0]) || !__builtin_isfinite(v[1]))
+    {
+      st->done = 1;
+      st->status = FEMTC_ERR_BREAKDOWN;
+    }
+    else if (v[1] <= st->tol2)
+    {
+      st->done = 1;
+      st->status = FEMTC_OK;
+    }
+    else if (st->it >= st->maxit)
+    {
+      st->done = 1;
+      st->status = FEMTC_ERR_NOT_CONVERGED;
+    }
+  }
+}
+
+// p = D^-1 r + beta p
+template <typename V>
+__global__ void __launch_bounds__(RB)
+    k_pupdate(V* __restrict__ p, const V* __restrict__ r,
+              const V* __restrict__ dinv, int64_t n,
+              const PcgState* __restrict__ st)
+{
+  if (st->done)
+    return;
+  const V b = (V)st->beta;
+  GRID_LOOP(i, n) { p[i] = dinv[i] * r[i] + b * p[i]; }
+}
+
+// ------------------------------------------------------- outer loop ----
+
+// partial sums of x.x
+__global__ void __launch_bounds__(RB)
+    k_norm2(const double* __restrict__ x, int64_t n, double* __restrict__ part)
+{
+  double s[1] = {0.0};
+  GRID_LOOP(i, n) { s[0] += x[i] * x[i]; }
+  store_partials<1>(s, part);
+}
+
+__global__ void __launch_bounds__(RB)
+    k_combine1(const double* __restrict__ part, double* out)
+{
+  double v[1];
+  combine_partials<1>(part, v);
+  if (threadIdx.x == 0)
+    *out = v[0];
+}
+
+// r = f - Au (FP64); partial sums of r.r
+__global__ void __launch_bounds__(RB)
+    k_residual(const double* __restrict__ f, const double* __restrict__ Au,
+               double* __restrict__ r, int64_t n, double* __restrict__ part)
+{
+  double s[1] = {0.0};
+  GRID_LOOP(i, n)
+  {
+    const double ri = f[i] - Au[i];
+    r[i] = ri;
+    s[0] += ri * ri;
+  }
+  store_partials<1>(s, part);
+}
+
+template <typename V>
+__global__ void __launch_bounds__(RB)
+    k_scale_to(const double* __restrict__ x, double s, V* __restrict__ y,
+               int64_t n)
+{
+  GRID_LOOP(i, n) { y[i] = (V)(x[i] * s); }
+}
+
+// u += s e
+template <typename V>
+__global__ void __launch_bounds__(RB)
+    k_correct(double* __restrict__ u, const V* __restrict__ e, double s,
+              int64_t n)
+{
+  GRID_LOOP(i, n) { u[i] += s * (double)e[i]; }
+}
+
+template <typename A, typename B>
+__global__ void __launch_bounds__(RB)
+    k_convert(const A* __restrict__ x, B* __restrict__ y, int64_t n)
+{
+  GRID_LOOP(i, n) { y[i] = (B)x[i]; }
+}
+
+__global__ void __launch_bounds__(RB)
+    k_invert(const double* __restrict__ d, double* __restrict__ dinv,
+             int64_t n, int* __restrict__ bad)
+{
+  GRID_LOOP(i, n)
+  {
+    const double v = d[i];
+    if (!(v > 0.0))
+      bad[0] = 1; // every writer stores the same value
+    dinv[i] = v > 0.0 ? 1.0 / v : 0.0;
+  }
+}
+
+// --------------------------------------------------------- diagonal ----
+//
+// diag(A) of the continuous operator, summed cell by cell:
+//   D_c[a][b][c] = sum_q sum_t F_t(q) X_t(a,qa) Y_t(b,qb) Z_t(c,qc)
+// with seven terms t (X, Y, Z each one of phi^2, g^2 or phi g):
+//   S00 (g^2,  phi^2, phi^2)   S11 (phi^2, g^2, phi^2)   S22 (phi^2, phi^2, g^2)
+//   2 S01 (phi g, phi g, phi^2)  2 S02 (phi g, phi^2, phi g)
+//   2 S12 (phi^2, phi g, phi g)  detJw (phi^2, phi^2, phi^2)
+// each evaluated by sum factorization (qc, then qb, then qa).  One workgroup
+// per cell; the tables and partial tensors live in LDS.  Setup kernel.
+__device__ inline double tab_kind(const double* ph, const double* gr, int kind,
+                                  int i)
+{
+  const double p = ph[i], g = gr[i];
+  return kind == 0 ? p * p : (kind == 1 ? g * g : p * g);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_diag(int op, int N, int Q, const T* __restrict__ tab,
+           const T* __restrict__ G, const T* __restrict__ detJw,
+           const int32_t* __restrict__ dofmap, int64_t num_cells,
+           double* __restrict__ diag)
+{
+  extern __shared__ double sm[];
+  double* ph = sm;
+  double* gr = ph + N * Q;
+  double* F = gr + N * Q;
+  double* T1 = F + Q * Q * Q;     // [qa][qb][c]
+  double* T2 = T1 + Q * Q * N;    // [qa][b][c]
+  const int64_t cell = blockIdx.x;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int Q3 = Q * Q * Q, N3 = N * N * N, NN = N * N;
+  for (int i = tid; i < 2 * N * Q; i += nt)
+    sm[i] = (double)tab[i];
+  // term -> (G slot, factor, X, Y, Z); slot 6 = detJw
+  const int slot[7] = {5, 3, 0, 4, 2, 1, 6};
+  const double fac[7] = {1, 1, 1, 2, 2, 2, 1};
+  const int kx[7] = {1, 0, 0, 2, 2, 0, 0};
+  const int ky[7] = {0, 1, 0, 2, 0, 2, 0};
+  const int kz[7] = {0, 0, 1, 0, 2, 2, 0};
+  double acc[3] = {0.0, 0.0, 0.0}; // dofs tid, tid+nt, tid+2nt (N3 <= 729)
+#pragma unroll
+  for (int t = 0; t < 7; ++t)
+  {
+    if (t < 6 && op == FEMTC_OP_MASS)
+      continue;
+    if (t == 6 && op == FEMTC_OP_STIFFNESS)
+      continue;
+    __syncthreads(); // previous term's readers are done
+    for (int q = tid; q < Q3; q += nt)
+      F[q] = fac[t] * (slot[t] == 6 ? (double)detJw[cell * Q3 + q]
+                                    : (double)G[(cell * Q3 + q) * 6 + slot[t]]);
+    __syncthreads();
+    for (int i = tid; i < Q * Q * N; i += nt)
+    {
+      const int c = i % N, qab = i / N;
+      double s = 0.0;
+      for (int qc = 0; qc < Q; ++qc)
+        s += F[qab * Q + qc] * tab_kind(ph, gr, kz[t], c * Q + qc);
+      T1[i] = s;
+    }
+    __syncthreads();
+    for (int i = tid; i < Q * NN; i += nt)
+    {
+      const int c = i % N, b = (i / N) % N, qa = i / NN;
+      double s = 0.0;
+      for (int qb = 0; qb < Q; ++qb)
+        s += T1[(qa * Q + qb) * N + c] * tab_kind(ph, gr, ky[t], b * Q + qb);
+      T2[i] = s;
+    }
+    __syncthreads();
+    for (int k = 0; k < 3; ++k)
+    {
+      const int i = tid + k * nt;
+      if (i < N3)
+      {
+        const int bc = i % NN, a = i / NN;
+        double s = 0.0;
+        for (int qa = 0; qa < Q; ++qa)
+          s += T2[qa * NN + bc] * tab_kind(ph, gr, kx[t], a * Q + qa);
+        acc[k] += s;
+      }
+    }
+  }
+  for (int k = 0; k < 3; ++k)
+  {
+    const int i = tid + k * nt;
+    if (i < N3)
+      unsafeAtomicAdd(diag + dofmap[cell * N3 + i], acc[k]);
+  }
+}
+
+size_t diag_lds_bytes(int N, int Q)
+{
+  return sizeof(double) * (size_t)(2 * N * Q + Q * Q * Q + Q * Q * N + Q * N * N);
+}
+
+int launch_diag(int op, int dtype, int N, int Q, const void* tab,
+                const void* G, const void* detJw, const int32_t* dofmap,
+                int64_t num_cells, double* diag, int64_t ndofs,
+                hipStream_t s)
+{
+  int st = fill_zero(diag, ndofs, FEMTC_F64, s);
+  if (st != FEMTC_OK)
+    return st;
+  if (num_cells == 0)
+    return FEMTC_OK;
+  if (num_cells > 0x7fffffffLL)
+    return FEMTC_ERR_INVALID_ARGUMENT;
+  const size_t lds = diag_lds_bytes(N, Q);
+  if (dtype == FEMTC_F64)
+    hipLaunchKernelGGL(k_diag<double>, dim3((unsigned)num_cells), dim3(256),
+                       lds, s, op, N, Q, (const double*)tab,
+                       (const double*)G, (const double*)detJw, dofmap,
+                       num_cells, diag);
+  else
+    hipLaunchKernelGGL(k_diag<float>, dim3((unsigned)num_cells), dim3(256),
+                       lds, s, op, N, Q, (const float*)tab, (const float*)G,
+                       (const float*)detJw, dofmap, num_cells, diag);
+  return hip_status(hipGetLastError());
+}
+
+// ----------------------------------------------------------- solver ----
+
+template <typename V>
+constexpr int dtype_of()
+{
+  return sizeof(V) == 8 ? FEMTC_F64 : FEMTC_F32;
+}
+
+} // namespace
+} // namespace femtc
+
+using namespace femtc;
+
+// The solver object behind the opaque femtc_solver handle.
+struct femtc_solver
+{
+  femtc_problem prob{};
+  int inner_dtype = FEMTC_F64;
+  int N = 0, Q = 0;
+  int64_t n = 0;
+  hipStream_t stream = nullptr;
+  // FP64
+  double* diag = nullptr;
+  double* dinv64 = nullptr;
+  double* r64 = nullptr;
+  double* Au64 = nullptr;
+  double* part = nullptr;
+  double* dscal = nullptr; // device scalar for norms
+  int* dbad = nullptr;
+  PcgState* st = nullptr;
+  PcgState* hst = nullptr; // pinned host copy
+  double* hscal = nullptr; // pinned host scalar
+  // inner precision (FP64 aliases the FP64 buffers)
+  void* rhs = nullptr;
+  void* e = nullptr;
+  void* r = nullptr;
+  void* p = nullptr;
+  void* Ap = nullptr;
+  void* dinv = nullptr;
+  void* tab = nullptr;   // inner-precision tables and geometry
+  void* G = nullptr;
+  void* detJw = nullptr;
+  int check_every = 16;
+};
+
+namespace femtc
+{
+namespace
+{
+
+int dev_alloc(void** p, size_t bytes)
+{
+  if (bytes == 0)
+    bytes = 8;
+  if (hipMalloc(p, bytes) != hipSuccess)
+  {
+    (void)hipGetLastError();
+    *p = nullptr;
+    return fail(FEMTC_ERR_OUT_OF_MEMORY,
+                "solver workspace allocation failed (" +
+                    std::to_string(bytes) + " bytes)");
+  }
+  return FEMTC_OK;
+}
+
+void dev_free(void* p)
+{
+  if (p)
+    (void)hipFree(p);
+}
+
+void destroy(femtc_solver* S)
+{
+  if (!S)
+    return;
+  const bool own_inner = S->inner_dtype != FEMTC_F64;
+  dev_free(S->diag);
+  dev_free(S->dinv64);
+  dev_free(S->r64);
+  dev_free(S->Au64);
+  dev_free(S->part);
+  dev_free(S->dscal);
+  dev_free(S->dbad);
+  dev_free(S->st);
+  if (S->hst)
+    (void)hipHostFree(S->hst);
+  if (S->hscal)
+    (void)hipHostFree(S->hscal);
+  dev_free(S->rhs);
+  dev_free(S->e);
+  dev_free(S->p);
+  dev_free(S->Ap);
+  if (own_inner)
+  {
+    dev_free(S->r);
+    dev_free(S->dinv);
+    dev_free(S->tab);
+    dev_free(S->G);
+    dev_free(S->detJw);
+  }
+  delete S;
+}
+
+ApplyReq operator_req(const femtc_solver* S, int dtype, const void* tab,
+                      const void* G, const void* detJw, const void* x,
+                      void* y)
+{
+  ApplyReq a;
+  a.op = S->prob.op;
+  a.dtype = dtype;
+  a.N = S->N;
+  a.Q = S->Q;
+  a.gather = 1;
+  a.tab = tab;
+  a.x = x;
+  a.y = y;
+  a.G = G;
+  a.detJw = detJw;
+  a.dofmap = S->prob.dofmap;
+  a.num_cells = S->prob.num_cells;
+  a.stream = S->stream;
+  return a;
+}
+
+int sync_read(femtc_solver* S, void* host, const void* dev, size_t bytes)
+{
+  if (hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, S->stream) !=
+          hipSuccess ||
+      hipStreamSynchronize(S->stream) != hipSuccess)
+    return fail(FEMTC_ERR_HIP, std::string("solver: ") +
+                                   hipGetErrorString(hipGetLastError()));
+  return FEMTC_OK;
+}
+
+#define LAUNCH_CHECK()                                                        \
+  do                                                                          \
+  {                                                                           \
+    hipError_t e_ = hipGetLastError();                                        \
+    if (e_ != hipSuccess)                                                     \
+      return fail(FEMTC_ERR_HIP,                                              \
+                  std::string("solver kernel launch: ") +                     \
+                      hipGetErrorString(e_));                                 \
+  } while (0)
+
+// PCG in precision V on A e = rhs (inner buffers of S), from e = 0.
+template <typename V>
+int pcg_run(femtc_solver* S, double rtol, int maxit, int* iters, int* status)
+{
+  hipStream_t s = S->stream;
+  const int64_t n = S->n;
+  V* e = (V*)S->e;
+  V* r = (V*)S->r;
+  V* p = (V*)S->p;
+  V* Ap = (V*)S->Ap;
+  const V* dinv = (const V*)S->dinv;
+  hipLaunchKernelGGL(k_start<V>, dim3(NBLK), dim3(RB), 0, s, (const V*)S->rhs,
+                     e, r, p, Ap, dinv, n, S->part);
+  hipLaunchKernelGGL(k_reduce_start, dim3(1), dim3(RB), 0, s,
+                     (const double*)S->part, S->st, rtol, maxit);
+  LAUNCH_CHECK();
+  ApplyReq a = operator_req(S, dtype_of<V>(), S->tab, S->G, S->detJw, p, Ap);
+  a.skip = &S->st->done;
+  for (;;)
+  {
+    for (int k = 0; k < S->check_every; ++k)
+    {
+      int st = run(a);
+      if (st != FEMTC_OK)
+        return st;
+      hipLaunchKernelGGL(k_dot<V>, dim3(NBLK), dim3(RB), 0, s, (const V*)p,
+                         (const V*)Ap, n, S->part, (const PcgState*)S->st);
+      hipLaunchKernelGGL(k_alpha, dim3(1), dim3(RB), 0, s,
+                         (const double*)S->part, S->st);
+      hipLaunchKernelGGL(k_update<V>, dim3(NBLK), dim3(RB), 0, s, e,
+                         (const V*)p, r, Ap, dinv, n, (const PcgState*)S->st,
+                         S->part);
+      hipLaunchKernelGGL(k_beta, dim3(1), dim3(RB), 0, s,
+                         (const double*)S->part, S->st);
+      hipLaunchKernelGGL(k_pupdate<V>, dim3(NBLK), dim3(RB), 0, s, p,
+                         (const V*)r, dinv, n, (const PcgState*)S->st);
+      LAUNCH_CHECK();
+    }
+    int st = sync_read(S, S->hst, S->st, sizeof(PcgState));
+    if (st != FEMTC_OK)
+      return st;
+    if (S->hst->done)
+      break;
+  }
+  *iters = S->hst->it;
+  *status = S->hst->status;
+  return FEMTC_OK;
+}
+
+// ||f - A u|| in FP64; leaves the residual in S->r64.
+int true_residual(femtc_solver* S, const double* f, const double* u,
+                  double* rnorm)
+{
+  hipStream_t s = S->stream;
+  int st = fill_zero(S->Au64, S->n, FEMTC_F64, s);
+  if (st != FEMTC_OK)
+    return st;
+  ApplyReq a = operator_req(S, FEMTC_F64, S->prob.tables, S->prob.G,
+                            S->prob.detJw, u, S->Au64);
+  st = run(a);
+  if (st != FEMTC_OK)
+    return st;
+  hipLaunchKernelGGL(k_residual, dim3(NBLK), dim3(RB), 0, s, f,
+                     (const double*)S->Au64, S->r64, S->n, S->part);
+  hipLaunchKernelGGL(k_combine1, dim3(1), dim3(RB), 0, s,
+                     (const double*)S->part, S->dscal);
+  LAUNCH_CHECK();
+  st = sync_read(S, S->hscal, S->dscal, sizeof(double));
+  if (st != FEMTC_OK)
+    return st;
+  *rnorm = std::sqrt(*S->hscal);
+  return FEMTC_OK;
+}
+
+int norm2(femtc_solver* S, const double* x, double* out)
+{
+  hipLaunchKernelGGL(k_norm2, dim3(NBLK), dim3(RB), 0, S->stream, x, S->n,
+                     S->part);
+  hipLaunchKernelGGL(k_combine1, dim3(1), dim3(RB), 0, S->stream,
+                     (const double*)S->part, S->dscal);
+  LAUNCH_CHECK();
+  int st = sync_read(S, S->hscal, S->dscal, sizeof(double));
+  if (st != FEMTC_OK)
+    return st;
+  *out = std::sqrt(*S->hscal);
+  return FEMTC_OK;
+}
+
+int check_problem(const femtc_problem* P)
+{
+  if (!P)
+    return fail(FEMTC_ERR_INVALID_ARGUMENT, "problem is NULL");
+  if (P->op < FEMTC_OP_MASS || P->op > FEMTC_OP_KPM)
+    return fail(FEMTC_ERR_INVALID_ARGUMENT, "op must be a femtc_op");
+  int st = check_shape(P->degree, P->nq, FEMTC_MODE_CORRECT);
+  if (st != FEMTC_OK)
+    return st;
+  if (P->num_cells < 0 || P->ndofs < 0)
+    return fail(FEMTC_ERR_INVALID_ARGUMENT, "negative size");
+  if (!P->tables || (P->num_cells > 0 && !P->dofmap))
+    return fail(FEMTC_ERR_INVALID_ARGUMENT, "tables and dofmap are required");
+  if (P->op != FEMTC_OP_MASS && !P->G)
+    return fail(FEMTC_ERR_INVALID_ARGUMENT, "G is required for this op");
+  if (P->op != FEMTC_OP_STIFFNESS && !P->detJw)
+    return fail(FEMTC_ERR_INVALID_ARGUMENT, "detJw is required for this op");
+  return FEMTC_OK;
+}
+
+} // namespace
+} // namespace femtc
+
+extern "C" {
+
+int femtc_global_diagonal(int op, int dtype, int degree, int nq,
+                          const void* tables, double* diag, int64_t ndofs,
+                          const int32_t* dofmap, const void* G,
+                          const void* detJw, int64_t num_cells, void* stream)
+{
+  int st = check_shape(degree, nq, FEMTC_MODE_CORRECT);
+  if (st != FEMTC_OK)
+    return st;
+  if ((st = check_dtype(dtype)) != FEMTC_OK)
+    return st;
+  if (op < FEMTC_OP_MASS || op > FEMTC_OP_KPM)
+    return fail(FEMTC_ERR_INVALID_ARGUMENT, "op must be a femtc_op");
+  if (!diag || !tables || (num_cells > 0 && !dofmap) || ndofs < 0 ||
+      num_cells < 0)
+    return fail(FEMTC_ERR_INVALID_ARGUMENT, "missing buffer or bad size");
+  if (op != FEMTC_OP_MASS && !G)
+    return fail(FEMTC_ERR_INVALID_ARGUMENT, "G is required for this op");
+  if (op != FEMTC_OP_STIFFNESS && !detJw)
+    return fail(FEMTC_ERR_INVALID_ARGUMENT, "detJw is required for this op");
+  st = launch_diag(op, dtype, degree + 1, nq, tables, G, detJw, dofmap,
+                   num_cells, diag, ndofs, (hipStream_t)stream);
+  if (st == FEMTC_ERR_HIP)
+    return fail(st, std::string("diagonal kernel: ") +
+                        hipGetErrorString(hipGetLastError()));
+  return st;
+}
+
+int femtc_solver_create(femtc_solver** out, const femtc_problem* prob,
+                        int inner_dtype, void* stream)
+{
+  if (!out)
+    return fail(FEMTC_ERR_INVALID_ARGUMENT, "out is NULL");
+  *out = nullptr;
+  int st = check_problem(prob);
+  if (st != FEMTC_OK)
+    return st;
+  if (inner_dtype != FEMTC_F64 && inner_dtype != FEMTC_F32)
+    return fail(FEMTC_ERR_INVALID_ARGUMENT,
+                "inner_dtype must be FEMTC_F64 (plain PCG) or FEMTC_F32 "
+                "(FP32 inner PCG with FP64 defect correction)");
+  femtc_solver* S = new femtc_solver;
+  S->prob = *prob;
+  S->inner_dtype = inner_dtype;
+  S->N = prob->degree + 1;
+  S->Q = prob->nq;
+  S->n = prob->ndofs;
+  S->stream = (hipStream_t)stream;
+  const int64_t n = S->n;
+  const int64_t C = prob->num_cells;
+  const int64_t Q3 = (int64_t)S->Q * S->Q * S->Q;
+  const size_t vs = inner_dtype == FEMTC_F64 ? 8 : 4;
+  hipStream_t s = S->stream;
+  do
+  {
+    if ((st = dev_alloc((void**)&S->diag, 8 * n)) ||
+        (st = dev_alloc((void**)&S->dinv64, 8 * n)) ||
+        (st = dev_alloc((void**)&S->r64, 8 * n)) ||
+        (st = dev_alloc((void**)&S->Au64, 8 * n)) ||
+        (st = dev_alloc((void**)&S->part, 8 * 2 * NBLK)) ||
+        (st = dev_alloc((void**)&S->dscal, 8)) ||
+        (st = dev_alloc((void**)&S->dbad, sizeof(int))) ||
+        (st = dev_alloc((void**)&S->st, sizeof(PcgState))) ||
+        (st = dev_alloc(&S->rhs, vs * n)) || (st = dev_alloc(&S->e, vs * n)) ||
+        (st = dev_alloc(&S->p, vs * n)) || (st = dev_alloc(&S->Ap, vs * n)))
+      break;
+    if (hipHostMalloc((void**)&S->hst, sizeof(PcgState)) != hipSuccess ||
+        hipHostMalloc((void**)&S->hscal, sizeof(double)) != hipSuccess)
+    {
+      st = fail(FEMTC_ERR_OUT_OF_MEMORY, "pinned host allocation failed");
+      break;
+    }
+    // Jacobi preconditioner from the FP64 operator
+    st = launch_diag(prob->op, FEMTC_F64, S->N, S->Q, prob->tables, prob->G,
+                     prob->detJw, prob->dofmap, C, S->diag, n, s);
+    if (st != FEMTC_OK)
+      break;
+    if ((st = fill_zero(S->dbad, 1, FEMTC_F32, s)) != FEMTC_OK)
+      break;
+    hipLaunchKernelGGL(k_invert, dim3(NBLK), dim3(RB), 0, s,
+                       (const double*)S->diag, S->dinv64, n, S->dbad);
+    if (inner_dtype == FEMTC_F64)
+    {
+      S->r = S->r64;
+      S->dinv = S->dinv64;
+      S->tab = (void*)prob->tables;
+      S->G = (void*)prob->G;
+      S->detJw = (void*)prob->detJw;
+    }
+    else
+    {
+      if ((st = dev_alloc(&S->r, 4 * n)) || (st = dev_alloc(&S->dinv, 4 * n)) ||
+          (st = dev_alloc(&S->tab, 4 * 2 * S->N * S->Q)))
+        break;
+      hipLaunchKernelGGL((k_convert<double, float>), dim3(NBLK), dim3(RB), 0,
+                         s, (const double*)S->dinv64, (float*)S->dinv, n);
+      hipLaunchKernelGGL((k_convert<double, float>), dim3(NBLK), dim3(RB), 0,
+                         s, prob->tables, (float*)S->tab,
+                         (int64_t)2 * S->N * S->Q);
+      if (prob->G)
+      {
+        if ((st = dev_alloc(&S->G, 4 * 6 * Q3 * C)))
+          break;
+        hipLaunchKernelGGL((k_convert<double, float>), dim3(NBLK), dim3(RB),
+                           0, s, prob->G, (float*)S->G, 6 * Q3 * C);
+      }
+      if (prob->detJw)
+      {
+        if ((st = dev_alloc(&S->detJw, 4 * Q3 * C)))
+          break;
+        hipLaunchKernelGGL((k_convert<double, float>), dim3(NBLK), dim3(RB),
+                           0, s, prob->detJw, (float*)S->detJw, Q3 * C);
+      }
+    }
+    hipError_t he = hipGetLastError();
+    if (he != hipSuccess)
+    {
+      st = fail(FEMTC_ERR_HIP, std::string("solver setup: ") +
+                                   hipGetErrorString(he));
+      break;
+    }
+    int bad = 0;
+    if ((st = sync_read(S, &bad, S->dbad, sizeof(int))) != FEMTC_OK)
+      break;
+    if (bad)
+    {
+      st = fail(FEMTC_ERR_INVALID_ARGUMENT,
+                "the operator diagonal has a non-positive entry (a dof that "
+                "no cell touches, or geometry that is not positive definite)");
+      break;
+    }
+  } while (false);
+  if (st != FEMTC_OK)
+  {
+    destroy(S);
+    return st;
+  }
+  *out = S;
+  return FEMTC_OK;
+}
+
+int femtc_solver_destroy(femtc_solver* solver)
+{
+  destroy(solver);
+  return FEMTC_OK;
+}
+
+int femtc_solver_diagonal(const femtc_solver* solver, const double** diag)
+{
+  if (!solver || !diag)
+    return fail(FEMTC_ERR_INVALID_ARGUMENT, "NULL argument");
+  *diag = solver->diag;
+  return FEMTC_OK;
+}
+
+int femtc_solver_solve(femtc_solver* S, const double* f, double* u,
+                       double rtol, double inner_rtol, int maxit,
+                       femtc_solve_info* info)
+{
+  if (!S || !f || !u)
+    return fail(FEMTC_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (!(rtol > 0.0) || maxit < 1)
+    return fail(FEMTC_ERR_INVALID_ARGUMENT, "rtol must be > 0, maxit >= 1");
+  const auto t0 = std::chrono::steady_clock::now();
+  hipStream_t s = S->stream;
+  const int64_t n = S->n;
+  femtc_solve_info inf{};
+  inf.status = FEMTC_OK;
+  double fn = 0.0, rn_final = 0.0;
+  int st = norm2(S, f, &fn);
+  if (st != FEMTC_OK)
+    return st;
+  if (S->inner_dtype == FEMTC_F64)
+  {
+    // plain PCG in FP64: rhs = f, solution into u
+    if (hipMemcpyAsync(S->rhs, f, 8 * n, hipMemcpyDeviceToDevice, s) !=
+        hipSuccess)
+      return fail(FEMTC_ERR_HIP, "copy of f failed");
+    int it = 0, pst = FEMTC_OK;
+    st = pcg_run<double>(S, rtol, maxit, &it, &pst);
+    if (st != FEMTC_OK)
+      return st;
+    if (hipMemcpyAsync(u, S->e, 8 * n, hipMemcpyDeviceToDevice, s) !=
+        hipSuccess)
+      return fail(FEMTC_ERR_HIP, "copy of u failed");
+    inf.inner_iterations = it;
+    inf.outer_iterations = 1;
+    inf.status = pst;
+    // report the true FP64 residual of the returned u
+    if ((st = true_residual(S, f, u, &rn_final)) != FEMTC_OK)
+      return st;
+  }
+  else
+  {
+    if (!(inner_rtol > 0.0 && inner_rtol < 1.0))
+      return fail(FEMTC_ERR_INVALID_ARGUMENT, "inner_rtol must be in (0, 1)");
+    if ((st = fill_zero(u, n, FEMTC_F64, s)) != FEMTC_OK)
+      return st;
+    double rn = fn; // r = f for u = 0
+    rn_final = rn;
+    if (hipMemcpyAsync(S->r64, f, 8 * n, hipMemcpyDeviceToDevice, s) !=
+        hipSuccess)
+      return fail(FEMTC_ERR_HIP, "copy of f failed");
+    const int max_outer = 60;
+    int stall = 0;
+    while (fn > 0.0 && rn > rtol * fn)
+    {
+      if (inf.outer_iterations >= max_outer || inf.inner_iterations >= maxit)
+      {
+        inf.status = FEMTC_ERR_NOT_CONVERGED;
+        break;
+      }
+      // inner: A_32 e = r / ||r||
+      hipLaunchKernelGGL(k_scale_to<float>, dim3(NBLK), dim3(RB), 0, s,
+                         (const double*)S->r64, 1.0 / rn, (float*)S->rhs, n);
+      LAUNCH_CHECK();
+      int it = 0, pst = FEMTC_OK;
+      st = pcg_run<float>(S, inner_rtol, maxit - inf.inner_iterations, &it,
+                          &pst);
+      if (st != FEMTC_OK)
+        return st;
+      inf.inner_iterations += it;
+      inf.outer_iterations += 1;
+      if (pst == FEMTC_ERR_BREAKDOWN)
+      {
+        inf.status = pst;
+        break;
+      }
+      hipLaunchKernelGGL(k_correct<float>, dim3(NBLK), dim3(RB), 0, s, u,
+                         (const float*)S->e, rn, n);
+      LAUNCH_CHECK();
+      double rn_new = 0.0;
+      if ((st = true_residual(S, f, u, &rn_new)) != FEMTC_OK)
+        return st;
+      stall = rn_new > 0.5 * rn ? stall + 1 : 0;
+      rn = rn_new;
+      rn_final = rn;
+      if (stall >= 3)
+      {
+        inf.status = FEMTC_ERR_NOT_CONVERGED; // refinement stagnates
+        break;
+      }
+    }
+  }
+  inf.relres = fn > 0.0 ? rn_final / fn : 0.0;
+  inf.seconds = std::chrono::duration<double>(
+                    std::chrono::steady_clock::now() - t0)
+                    .count();
+  if (info)
+    *info = inf;
+  if (inf.status != FEMTC_OK)
+    return fail(inf.status,
+                "solve stopped: " +
+                    std::string(femtc_status_string(inf.status)) +
+                    ", relative residual " + std::to_string(inf.relres));
+  return FEMTC_OK;
+}
+
+} // extern "C"

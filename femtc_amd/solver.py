@@ -1,0 +1,80 @@
+"""Device-resident Jacobi-PCG and FP64 defect correction (libfemtc solver).
+
+The reference has no solver; this is the solve path built on the operator:
+A = M, K or K+M applied through the dofmap, a Jacobi preconditioner from the
+sum-factorized diagonal, CG scalars kept on the device, and (inner=F32) an
+FP32 inner PCG wrapped in FP64 iterative refinement.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+from . import _lib
+from .operators import _dtype_code, _ptr, _stream
+
+
+def global_diagonal(op: int, tables, diag, dofmap, num_cells: int,
+                    degree: int, nq: int, G=None, detJw=None,
+                    stream=None) -> None:
+    """diag (FP64, overwritten) = diagonal of the continuous operator."""
+    _lib.check(_lib.lib().femtc_global_diagonal(
+        op, _dtype_code(tables), degree, nq, _ptr(tables), _ptr(diag),
+        diag.numel(), _ptr(dofmap), _ptr(G), _ptr(detJw), num_cells,
+        _stream(stream)), "global_diagonal")
+
+
+class PCGSolver:
+    """Solver for A u = f on one problem (device tensors, FP64).
+
+    inner=F64: plain PCG.  inner=F32: FP32 inner PCG (tolerance inner_rtol)
+    inside FP64 defect correction.  The tensors passed here must stay alive
+    and unchanged while the solver is used."""
+
+    def __init__(self, op: int, tables, dofmap, num_cells: int, degree: int,
+                 nq: int, ndofs: int, G=None, detJw=None, inner: int = _lib.F64,
+                 stream=None):
+        self._keep = (tables, dofmap, G, detJw)
+        self.ndofs = ndofs
+        self.inner = inner
+        self.problem = _lib.Problem(op, degree, nq, num_cells, ndofs,
+                                    _ptr(dofmap), _ptr(tables), _ptr(G),
+                                    _ptr(detJw))
+        h = C.c_void_p()
+        _lib.check(_lib.lib().femtc_solver_create(
+            C.byref(h), C.byref(self.problem), inner, _stream(stream)),
+            "solver_create")
+        self._h = h
+
+    def solve(self, f, u, rtol: float = 1e-10, inner_rtol: float = 1e-4,
+              maxit: int = 20000, check: bool = True) -> dict:
+        """u = A^-1 f to rtol.  Returns the solve info as a dict; raises on
+        failure unless check=False."""
+        if self._h is None:
+            raise RuntimeError("solver is closed")
+        if f.numel() != self.ndofs or u.numel() != self.ndofs:
+            raise ValueError("f and u must have ndofs entries")
+        info = _lib.SolveInfo()
+        st = _lib.lib().femtc_solver_solve(self._h, _ptr(f), _ptr(u), rtol,
+                                           inner_rtol, maxit, C.byref(info))
+        out = {k: getattr(info, k) for k, _ in _lib.SolveInfo._fields_}
+        out["status_string"] = _lib.status_string(st)
+        if check and st != _lib.OK:
+            raise _lib.FemtcError(st, "solver_solve", _lib.last_error())
+        return out
+
+    def close(self) -> None:
+        if self._h is not None:
+            _lib.lib().femtc_solver_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

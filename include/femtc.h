@@ -193,6 +193,66 @@ int femtc_dofmap_structured(int nx, int ny, int nz, int degree,
 int femtc_geometry_affine(int64_t num_cells, int nq, const double* J,
                           void* G, void* detJw, int dtype, void* stream);
 
+/* ---------------------------------------------------------------------- */
+/* Solver: Jacobi-preconditioned CG on the continuous operator, and FP64   */
+/* defect correction around an FP32 inner PCG.  The reference has no       */
+/* solver; these entry points have no counterpart there.                   */
+/* ---------------------------------------------------------------------- */
+/* diag = diagonal of the continuous operator op (overwritten, FP64),
+ * summed cell by cell with sum factorization; inputs of dtype F64 or F32. */
+int femtc_global_diagonal(int op, int dtype, int degree, int nq,
+                          const void* tables, double* diag, int64_t ndofs,
+                          const int32_t* dofmap, const void* G,
+                          const void* detJw, int64_t num_cells, void* stream);
+
+/* The operator of a solve: device buffers in FP64, layouts as above.  The
+ * caller keeps them alive and unchanged while a solver refers to them. */
+typedef struct femtc_problem
+{
+  int op; /* femtc_op */
+  int degree;
+  int nq;
+  int64_t num_cells;
+  int64_t ndofs;
+  const int32_t* dofmap;
+  const double* tables; /* [phi | dphi], 2*N*nq */
+  const double* G;      /* [num_cells][nq^3][6], or NULL for the mass op */
+  const double* detJw;  /* [num_cells][nq^3], or NULL for the stiffness op */
+} femtc_problem;
+
+typedef struct femtc_solve_info
+{
+  int status;           /* femtc_status of the solve */
+  int outer_iterations; /* defect-correction steps (1 for plain PCG) */
+  int inner_iterations; /* CG iterations, summed over the steps */
+  double relres;        /* ||f - A u||_2 / ||f||_2, evaluated in FP64 */
+  double seconds;       /* host wall time of the call */
+} femtc_solve_info;
+
+typedef struct femtc_solver femtc_solver; /* opaque */
+
+/* Allocate the workspace on the current device and build the Jacobi
+ * preconditioner from the FP64 operator.  inner_dtype FEMTC_F64: plain PCG
+ * in FP64.  FEMTC_F32: FP32 inner PCG (FP32 vectors, tables and geometry
+ * copy; FP64 dot products) inside FP64 defect correction.  The solver and
+ * every later solve run on `stream`.  Fails with FEMTC_ERR_INVALID_ARGUMENT
+ * when a diagonal entry is not positive (a dof no cell touches). */
+int femtc_solver_create(femtc_solver** solver, const femtc_problem* problem,
+                        int inner_dtype, void* stream);
+/* Solve A u = f (f, u FP64 device vectors of ndofs; u overwritten) to
+ * ||f - A u|| <= rtol ||f||.  Plain PCG stops on its recursive residual;
+ * defect correction stops on the FP64 residual recomputed after each inner
+ * solve (inner tolerance inner_rtol).  maxit caps the CG iterations (summed
+ * over inner solves).  Synchronous; info may be NULL.  Returns
+ * FEMTC_ERR_NOT_CONVERGED or FEMTC_ERR_BREAKDOWN (with info filled in) when
+ * the tolerance is not met. */
+int femtc_solver_solve(femtc_solver* solver, const double* f, double* u,
+                       double rtol, double inner_rtol, int maxit,
+                       femtc_solve_info* info);
+/* Device pointer to the solver's FP64 operator diagonal. */
+int femtc_solver_diagonal(const femtc_solver* solver, const double** diag);
+int femtc_solver_destroy(femtc_solver* solver);
+
 #ifdef __cplusplus
 }
 #endif

@@ -101,3 +101,26 @@ def test_argument_errors_without_device_work():
                                     W.ctypes.data, G.ctypes.data, 1, 3, 0, 0, 0, 1)
     assert st == _lib.ERR_INVALID_ARGUMENT
     assert "dphi" in _lib.last_error()
+
+
+def test_solver_create_rejects_bad_problems_without_touching_the_device():
+    # argument checks run before any HIP call
+    import ctypes as C_
+    L = F.lib()
+    h = C_.c_void_p()
+    p = _lib.Problem(F.OP_STIFFNESS, 3, 4, 10, 100, 1, 1, None, None)
+    assert L.femtc_solver_create(C_.byref(h), C_.byref(p), F.F64, None) == \
+        _lib.ERR_INVALID_ARGUMENT
+    assert "G is required" in _lib.last_error()
+    p = _lib.Problem(F.OP_MASS, 9, 10, 10, 100, 1, 1, None, 1)
+    assert L.femtc_solver_create(C_.byref(h), C_.byref(p), F.F64, None) == \
+        _lib.ERR_DEGREE_NOT_SUPPORTED
+    p = _lib.Problem(7, 3, 4, 10, 100, 1, 1, 1, 1)
+    assert L.femtc_solver_create(C_.byref(h), C_.byref(p), F.F64, None) == \
+        _lib.ERR_INVALID_ARGUMENT
+    p = _lib.Problem(F.OP_KPM, 3, 4, 10, 100, 1, 1, 1, 1)
+    assert L.femtc_solver_create(C_.byref(h), C_.byref(p), F.BF16, None) == \
+        _lib.ERR_INVALID_ARGUMENT
+    assert L.femtc_solver_create(None, C_.byref(p), F.F64, None) == \
+        _lib.ERR_INVALID_ARGUMENT
+    assert h.value is None

@@ -1,0 +1,218 @@
+"""GPU parity of the solve path: operator diagonal, FP64 Jacobi-PCG and FP64
+defect correction around the FP32 inner PCG, through the C ABI.
+
+Bars: the diagonal matches the oracle to 1e-12 (FP64) and, independently,
+e_i^T A e_i probed through the GPU operator itself; PCG solutions match the
+oracle's PCG solution to 1e-9 (both solved to rtol 1e-12); every returned
+residual is re-evaluated with the oracle's FP64 operator; refined solves
+reach the requested 1e-10 relative residual, and their error against the
+FP64 oracle solution is printed.
+"""
+import numpy as np
+import pytest
+
+import femtc_amd as F
+import oracle as O
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+
+def dev(a, dtype=torch.float64):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(device="cuda",
+                                                         dtype=dtype)
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.double().cpu().numpy()
+
+
+def rel(a, b):
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+def _box(nx, ny, nz, P, Q, seed):
+    C = nx * ny * nz
+    J = O.random_jacobians(seed, C)
+    G, Dw = O.geometry_affine(J, Q)
+    dm = O.dofmap_structured(nx, ny, nz, P)
+    ng = (nx * P + 1) * (ny * P + 1) * (nz * P + 1)
+    return C, G, Dw, dm, ng
+
+
+DIAG_BOXES = [(2, 3, 2, 1, 2), (3, 2, 2, 2, 3), (2, 2, 3, 3, 5), (2, 2, 2, 4, 5),
+              (2, 1, 2, 6, 7), (1, 2, 2, 7, 10), (1, 1, 2, 8, 9),
+              (1, 1, 1, 8, 11)]
+
+
+@pytest.mark.parametrize("nx,ny,nz,P,Q", DIAG_BOXES)
+@pytest.mark.parametrize("op", [F.OP_MASS, F.OP_STIFFNESS, F.OP_KPM])
+def test_global_diagonal_vs_oracle(op, nx, ny, nz, P, Q):
+    C, G, Dw, dm, ng = _box(nx, ny, nz, P, Q, 60 + P)
+    tab = O.basis_tables(P, Q)
+    ref = O.global_diagonal(op, P + 1, Q, tab, ng, dm, G, Dw)
+    d = torch.full((ng,), 9.0, device="cuda", dtype=torch.float64)
+    F.global_diagonal(op, dev(tab), d, dev(dm, torch.int32), C, P, Q,
+                      G=dev(G), detJw=dev(Dw))
+    assert rel(host(d), ref) <= 1e-12
+    # FP32 inputs, FP64 accumulation
+    f = torch.float32
+    d32 = torch.empty(ng, device="cuda", dtype=torch.float64)
+    F.global_diagonal(op, dev(tab, f), d32, dev(dm, torch.int32), C, P, Q,
+                      G=dev(G, f), detJw=dev(Dw, f))
+    assert rel(host(d32), ref) <= 1e-6
+
+
+@pytest.mark.parametrize("op", [F.OP_MASS, F.OP_STIFFNESS, F.OP_KPM])
+def test_diagonal_equals_operator_probed_with_unit_vectors(op):
+    # independent of the oracle: diag_i = e_i^T A e_i with the GPU operator
+    nx, ny, nz, P, Q = 2, 1, 2, 3, 5
+    C, G, Dw, dm, ng = _box(nx, ny, nz, P, Q, 77)
+    tab, ddm = dev(O.basis_tables(P, Q)), dev(dm, torch.int32)
+    dG, dD = dev(G), dev(Dw)
+    d = torch.empty(ng, device="cuda", dtype=torch.float64)
+    F.global_diagonal(op, tab, d, ddm, C, P, Q, G=dG, detJw=dD)
+    probe = np.empty(ng)
+    x = torch.zeros(ng, device="cuda", dtype=torch.float64)
+    y = torch.empty(ng, device="cuda", dtype=torch.float64)
+    for i in range(ng):
+        x.zero_()
+        x[i] = 1.0
+        F.global_apply(op, tab, x, y, ddm, C, P, Q, G=dG, detJw=dD)
+        probe[i] = y[i].item()
+    assert rel(host(d), probe) <= 1e-13
+
+
+def _problem(op, nx, ny, nz, P, Q, seed):
+    C, G, Dw, dm, ng = _box(nx, ny, nz, P, Q, seed)
+    tab = O.basis_tables(P, Q)
+    f = O.uniform(seed + 1, ng)
+    return C, G, Dw, dm, ng, tab, f
+
+
+SOLVE_CASES = [(F.OP_KPM, 3, 3, 3, 2, 3), (F.OP_KPM, 2, 3, 2, 4, 5),
+               (F.OP_MASS, 3, 2, 2, 3, 4), (F.OP_KPM, 2, 2, 2, 6, 7),
+               (F.OP_KPM, 2, 1, 2, 8, 9)]
+
+
+@pytest.mark.parametrize("op,nx,ny,nz,P,Q", SOLVE_CASES)
+def test_pcg_f64_vs_oracle(op, nx, ny, nz, P, Q):
+    C, G, Dw, dm, ng, tab, f = _problem(op, nx, ny, nz, P, Q, 90 + P)
+    name = {F.OP_MASS: "mass", F.OP_KPM: "kpm"}[op]
+    u_ref, it_ref, rel_ref = O.pcg(name, P + 1, Q, tab, f, dm, G, Dw,
+                                   rtol=1e-12)
+    G_d = dev(G) if op != F.OP_MASS else None
+    with F.PCGSolver(op, dev(tab), dev(dm, torch.int32), C, P, Q, ng, G=G_d,
+                     detJw=dev(Dw)) as S:
+        u = torch.full((ng,), 4.0, device="cuda", dtype=torch.float64)
+        info = S.solve(dev(f), u, rtol=1e-12)
+    uh = host(u)
+    # same algorithm in the same arithmetic: same iteration count +- 2
+    assert abs(info["inner_iterations"] - it_ref) <= 2, (info, it_ref)
+    assert rel(uh, u_ref) <= 1e-9
+    res = rel(O.global_operator(op, P + 1, Q, tab, uh, dm, G, Dw), f)
+    assert res <= 2e-12
+    assert abs(res - info["relres"]) <= 1e-3 * res + 1e-15
+
+
+@pytest.mark.parametrize("op,nx,ny,nz,P,Q", SOLVE_CASES)
+def test_refined_f32_reaches_1e10(op, nx, ny, nz, P, Q):
+    C, G, Dw, dm, ng, tab, f = _problem(op, nx, ny, nz, P, Q, 95 + P)
+    name = {F.OP_MASS: "mass", F.OP_KPM: "kpm"}[op]
+    u_ref, _, _ = O.pcg(name, P + 1, Q, tab, f, dm, G, Dw, rtol=1e-14)
+    G_d = dev(G) if op != F.OP_MASS else None
+    with F.PCGSolver(op, dev(tab), dev(dm, torch.int32), C, P, Q, ng, G=G_d,
+                     detJw=dev(Dw), inner=F.F32) as S:
+        u = torch.empty(ng, device="cuda", dtype=torch.float64)
+        info = S.solve(dev(f), u, rtol=1e-10, inner_rtol=1e-4)
+    uh = host(u)
+    res = rel(O.global_operator(op, P + 1, Q, tab, uh, dm, G, Dw), f)
+    err = rel(uh, u_ref)
+    print(f"refined P={P} Q={Q}: outer {info['outer_iterations']} inner "
+          f"{info['inner_iterations']} relres {res:.2e} error vs FP64 oracle "
+          f"{err:.2e}")
+    assert info["relres"] <= 1e-10
+    assert res <= 1.05e-10
+    assert err <= 1e-8
+    assert info["outer_iterations"] >= 2  # FP32 alone cannot reach 1e-10
+
+
+def test_solver_reuse_is_deterministic():
+    C, G, Dw, dm, ng, tab, f = _problem(F.OP_KPM, 3, 2, 3, 4, 5, 5)
+    with F.PCGSolver(F.OP_KPM, dev(tab), dev(dm, torch.int32), C, 4, 5, ng,
+                     G=dev(G), detJw=dev(Dw), inner=F.F32) as S:
+        df = dev(f)
+        u1 = torch.empty(ng, device="cuda", dtype=torch.float64)
+        u2 = torch.empty(ng, device="cuda", dtype=torch.float64)
+        i1 = S.solve(df, u1, rtol=1e-10)
+        i2 = S.solve(df, u2, rtol=1e-10)
+        # a different right-hand side in between leaves no trace
+        u3 = torch.empty(ng, device="cuda", dtype=torch.float64)
+        S.solve(dev(O.uniform(6, ng)), u3, rtol=1e-10)
+        i4 = S.solve(df, u3, rtol=1e-10)
+    assert i1["inner_iterations"] == i2["inner_iterations"] == i4["inner_iterations"]
+    # atomics in the scatter may change the last bits only
+    assert rel(host(u1), host(u2)) <= 1e-12
+    assert rel(host(u1), host(u3)) <= 1e-12
+
+
+def test_iteration_cap_reports_not_converged():
+    C, G, Dw, dm, ng, tab, f = _problem(F.OP_KPM, 3, 3, 3, 3, 4, 8)
+    with F.PCGSolver(F.OP_KPM, dev(tab), dev(dm, torch.int32), C, 3, 4, ng,
+                     G=dev(G), detJw=dev(Dw)) as S:
+        u = torch.empty(ng, device="cuda", dtype=torch.float64)
+        info = S.solve(dev(f), u, rtol=1e-12, maxit=3, check=False)
+        assert info["status"] == 8  # FEMTC_ERR_NOT_CONVERGED
+        assert info["inner_iterations"] == 3
+        with pytest.raises(F.FemtcError, match="did not converge"):
+            S.solve(dev(f), u, rtol=1e-12, maxit=3)
+        # the residual reported is the true one after 3 iterations
+        res = rel(O.global_operator(F.OP_KPM, 4, 4, tab, host(u), dm, G, Dw), f)
+        assert abs(res - info["relres"]) <= 1e-6 * res
+
+
+def test_zero_rhs_gives_zero_solution():
+    C, G, Dw, dm, ng, tab, f = _problem(F.OP_KPM, 2, 2, 2, 2, 3, 9)
+    for inner in (F.F64, F.F32):
+        with F.PCGSolver(F.OP_KPM, dev(tab), dev(dm, torch.int32), C, 2, 3, ng,
+                         G=dev(G), detJw=dev(Dw), inner=inner) as S:
+            u = torch.full((ng,), 1.0, device="cuda", dtype=torch.float64)
+            info = S.solve(torch.zeros(ng, device="cuda", dtype=torch.float64),
+                           u)
+            assert info["inner_iterations"] == 0
+            assert info["relres"] == 0.0
+            assert float(u.abs().max()) == 0.0
+
+
+def test_untouched_dof_is_rejected():
+    # a dof no cell touches has a zero diagonal: creation fails loudly
+    C, G, Dw, dm, ng, tab, f = _problem(F.OP_KPM, 2, 2, 2, 2, 3, 10)
+    with pytest.raises(F.FemtcError, match="non-positive"):
+        F.PCGSolver(F.OP_KPM, dev(tab), dev(dm, torch.int32), C, 2, 3, ng + 1,
+                    G=dev(G), detJw=dev(Dw))
+
+
+def test_refined_mid_size_p6_properties():
+    # 12^3 cells at P=6 (Q=7): size-independent checks only (FP64 residual,
+    # refined vs plain agreement)
+    n, P, Q = 12, 6, 7
+    C, G, Dw, dm, ng = _box(n, n, n, P, Q, 3)
+    tab = dev(O.basis_tables(P, Q))
+    f = dev(O.uniform(4, ng))
+    ddm, dG, dD = dev(dm, torch.int32), dev(G), dev(Dw)
+    u64 = torch.empty(ng, device="cuda", dtype=torch.float64)
+    u32 = torch.empty(ng, device="cuda", dtype=torch.float64)
+    with F.PCGSolver(F.OP_KPM, tab, ddm, C, P, Q, ng, G=dG, detJw=dD) as S:
+        i64 = S.solve(f, u64, rtol=1e-10)
+    with F.PCGSolver(F.OP_KPM, tab, ddm, C, P, Q, ng, G=dG, detJw=dD,
+                     inner=F.F32) as S:
+        i32 = S.solve(f, u32, rtol=1e-10)
+    print("plain", i64, "refined", i32)
+    assert i64["relres"] <= 1.01e-10 and i32["relres"] <= 1e-10
+    # residual recomputed independently with the GPU operator
+    y = torch.empty(ng, device="cuda", dtype=torch.float64)
+    F.global_apply(F.OP_KPM, tab, u32, y, ddm, C, P, Q, G=dG, detJw=dD)
+    assert float((f - y).norm() / f.norm()) <= 1.05e-10
+    assert float((u64 - u32).norm() / u64.norm()) <= 1e-8
