@@ -66,6 +66,16 @@ SIGNATURES = {
                                         _i32, _i32, _i32, _i32, _i32]),
     "femtc_dofmap_structured": (_i32, [_i32, _i32, _i32, _i32, _vp, _vp]),
     "femtc_geometry_affine": (_i32, [_i64, _i32, _vp, _vp, _vp, _i32, _vp]),
+    "femtc_plan_create": (_i32, [C.POINTER(_vp), _i32, _i32, _i32, _i32, _vp,
+                                 _i64, _vp]),
+    "femtc_plan_destroy": (_i32, [_vp]),
+    "femtc_plan_set_block": (_i32, [_vp, _i32, _i32, _i32, _i32, _i32, _i32]),
+    "femtc_plan_get_block": (_i32, [_vp, _i32, _i32, _i32, C.POINTER(_i32),
+                                    C.POINTER(_i32), C.POINTER(_i32)]),
+    "femtc_plan_apply": (_i32, [_vp, _i32, _i32, _i32, _vp, _vp, _vp, _vp, _vp,
+                                _i32, _vp]),
+    "femtc_plan_autotune": (_i32, [_vp, _i32, _i32, _i32, _vp, _vp, _vp, _vp,
+                                   _vp, _i32, _vp, C.POINTER(C.c_double)]),
     "femtc_global_diagonal": (_i32, [_i32, _i32, _i32, _i32, _vp, _vp, _i64,
                                      _vp, _vp, _vp, _i64, _vp]),
     "femtc_solver_create": (_i32, [C.POINTER(_vp), _vp, _i32, _vp]),

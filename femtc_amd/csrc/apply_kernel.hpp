@@ -171,17 +171,61 @@ constexpr int work_lds_elems()
   return WORK > GELEM ? WORK : GELEM;
 }
 
-template <typename T, int N, int Q, int OP, bool REF, bool GATHER>
-__global__ void __launch_bounds__(block_threads<Q>())
+// Synchronisation between the contraction stages.  A one-wave workgroup (or
+// a multi-wave one whose cells span waves) uses the workgroup barrier; in
+// the blocked scatter every wave owns whole cells and its own LDS work area,
+// so a stage only has to order the wave's own LDS accesses: LDS operations of
+// one wave execute in issue order, and the fences plus wave barrier keep the
+// compiler from moving accesses across the stage boundary.
+template <bool WAVE_LOCAL>
+__device__ inline void stage_sync()
+{
+  if constexpr (WAVE_LOCAL)
+  {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+  else
+    __syncthreads();
+}
+
+// Bytes of dynamic LDS of a blocked-scatter workgroup: BW per-wave work
+// areas, then the block's accumulator, global-index and count arrays.
+template <typename T, int N, int Q, int OP>
+constexpr size_t blocked_work_bytes()
+{
+  return sizeof(Piece<T>) * (size_t)((work_lds_elems<T, N, Q, OP>() + 1) / 2);
+}
+
+inline size_t blocked_box(int P, int cbx, int cby, int cbz)
+{
+  return (size_t)(cbx * P + 1) * (cby * P + 1) * (cbz * P + 1);
+}
+
+template <typename T>
+inline size_t blocked_box_bytes(size_t box)
+{
+  return box * (sizeof(T) + 2 * sizeof(int32_t));
+}
+
+// BW = 0: one workgroup per CPB consecutive cells, direct scatter.
+// BW > 0: blocked scatter, BW waves per workgroup and one block of
+// cbx x cby x cbz cells per workgroup (Q <= 8 and cbz % CPB == 0).
+template <typename T, int N, int Q, int OP, bool REF, bool GATHER, int BW>
+__global__ void __launch_bounds__(BW ? 64 * BW : block_threads<Q>())
     apply_cells(const T* __restrict__ tab, const T* __restrict__ x,
                 T* __restrict__ y, const T* __restrict__ G,
                 const T* __restrict__ detJw, const int32_t* __restrict__ dofmap,
-                long long num_cells, const int* __restrict__ skip)
+                long long num_cells, const int* __restrict__ skip,
+                BlockArgs ba)
 {
   if (skip && *skip)
     return;
+  constexpr bool BLK = BW > 0;
+  static_assert(!BLK || (GATHER && Q * Q <= 64), "blocked scatter layout");
   constexpr int CPB = cells_per_block<Q>();
-  constexpr int BT = block_threads<Q>();
+  constexpr int BT = BLK ? 64 : block_threads<Q>(); // threads per cell group
   constexpr int NL = Q * Q;
   constexpr bool LIT = (OP == OP_STIFF_LIT);
   constexpr bool MASSTERM = (OP == OP_MASS || OP == OP_KPM);
@@ -219,18 +263,53 @@ __global__ void __launch_bounds__(block_threads<Q>())
   constexpr int Y_RA = 0, Y_RB = 1, Y_RC = 2;  // B3 -> B2
   constexpr int X_S0 = 0, X_SB = 1;            // B2 -> B1
 
-  __shared__ Piece<T> Ls[LDSW];
+  __shared__ Piece<T> Ls_static[BLK ? 1 : LDSW];
+  extern __shared__ __align__(16) unsigned char femtc_lds_dyn[];
+  Piece<T>* Ls_dyn = reinterpret_cast<Piece<T>*>(femtc_lds_dyn);
+  // this wave's (or, unblocked, the workgroup's) work area
+  const int wave = BLK ? (int)(threadIdx.x >> 6) : 0;
+  Piece<T>* Ls = BLK ? Ls_dyn + wave * LDSW : Ls_static;
   T* L = reinterpret_cast<T*>(Ls);
 
   const T* __restrict__ ph = tab;
   const T* __restrict__ gr = tab + N * Q;
 
-  const int tid = threadIdx.x;
+  const int tid = BLK ? (int)(threadIdx.x & 63) : (int)threadIdx.x;
   const int lc = tid / NL;
   const int ln = tid - lc * NL;
   const bool lane_ok = lc < CPB;
   T* Xc = L + (lane_ok ? lc : 0) * WCELL;
   T* Yc = Xc;
+
+  // blocked scatter: the block's accumulator, global index and count of
+  // every dof of its (cbx P+1)(cby P+1)(cbz P+1) box
+  constexpr int P = N - 1;
+  T* acc = nullptr;
+  int32_t* gid = nullptr;
+  int32_t* cnt = nullptr;
+  int BY = 0, BZ = 0, BOX = 0;
+  long long blk_ci = 0, blk_cj = 0, blk_ck = 0; // the block's first cell
+  if constexpr (BLK)
+  {
+    BY = ba.cby * P + 1;
+    BZ = ba.cbz * P + 1;
+    BOX = (ba.cbx * P + 1) * BY * BZ;
+    acc = reinterpret_cast<T*>(Ls_dyn + (blockDim.x >> 6) * LDSW);
+    gid = reinterpret_cast<int32_t*>(acc + BOX);
+    cnt = gid + BOX;
+    for (int s = threadIdx.x; s < BOX; s += blockDim.x)
+    {
+      acc[s] = T(0);
+      cnt[s] = 0;
+    }
+    const long long b = blockIdx.x;
+    const long long bk = b % ba.nbz;
+    const long long bj = (b / ba.nbz) % ba.nby;
+    const long long bi = b / ((long long)ba.nbz * ba.nby);
+    blk_ci = bi * ba.cbx;
+    blk_cj = bj * ba.cby;
+    blk_ck = bk * ba.cbz;
+  }
 
   // thread roles
   const int r1a = ln / N, r1c = ln - (ln / N) * N;  // F1/B1: (a, c)
@@ -241,11 +320,32 @@ __global__ void __launch_bounds__(block_threads<Q>())
   const int uoff = r1a * NN + r1c; // + b*N within a cell
 
   {
-    const long long cell0 = (long long)blockIdx.x * CPB;
+    // the CPB consecutive cells of this wave (blocked) or workgroup
+    long long cell0 = 0, ncells_blk = 0;
+    int bl_i = 0, bl_j = 0, bl_k = 0; // blocked: local cell of lane group 0
+    if constexpr (BLK)
+    {
+      const int lcb0 = wave * CPB;
+      bl_k = lcb0 % ba.cbz;
+      bl_j = (lcb0 / ba.cbz) % ba.cby;
+      bl_i = lcb0 / (ba.cbz * ba.cby);
+      const long long ci = blk_ci + bl_i, cj = blk_cj + bl_j,
+                      ck0 = blk_ck + bl_k;
+      const long long left = ba.ncz - ck0;
+      if (lcb0 < ba.cbx * ba.cby * ba.cbz && ci < ba.ncx && cj < ba.ncy &&
+          left > 0)
+      {
+        ncells_blk = left < CPB ? left : (long long)CPB;
+        cell0 = (ci * ba.ncy + cj) * ba.ncz + ck0;
+      }
+    }
+    else
+    {
+      cell0 = (long long)blockIdx.x * CPB;
+      ncells_blk = num_cells - cell0 < CPB ? num_cells - cell0 : (long long)CPB;
+    }
     const long long cell = cell0 + lc;
-    const bool live = lane_ok && cell < num_cells;
-    const long long ncells_blk =
-        num_cells - cell0 < CPB ? num_cells - cell0 : (long long)CPB;
+    const bool live = lane_ok && lc < ncells_blk;
 
     // ---------------- global loads issued up front ------------------------
     T u[N];
@@ -283,6 +383,22 @@ __global__ void __launch_bounds__(block_threads<Q>())
       }
     }
 
+    // blocked: record which global dof sits in each box slot (every cell
+    // touching a slot writes the same index when the dofmap is the
+    // structured one; otherwise the scatter below falls back to atomics)
+    int slot0 = 0; // box slot of (a, 0, c) of this thread's cell
+    if constexpr (BLK)
+    {
+      slot0 = ((bl_i * P + r1a) * BY + bl_j * P) * BZ + (bl_k + lc) * P + r1c;
+      if (live && role1)
+      {
+#pragma unroll
+        for (int b = 0; b < N; ++b)
+          gid[slot0 + b * BZ] = idx[b];
+      }
+      __syncthreads(); // box cleared and indexed for the whole block
+    }
+
     // ---------------- F1: contract axis b ---------------------------------
     if (live && role1)
     {
@@ -310,7 +426,7 @@ __global__ void __launch_bounds__(block_threads<Q>())
           Xc[X_G * SX + q * XQ + r1c * XC + r1a] = ag[q];
       }
     }
-    __syncthreads();
+    stage_sync<BLK>();
 
     // ---------------- F2: contract axis c ---------------------------------
     T bpp[Q], bpg[Q], bgp[Q];
@@ -343,7 +459,7 @@ __global__ void __launch_bounds__(block_threads<Q>())
         }
       }
     }
-    __syncthreads(); // X is read; Y overwrites the same LDS
+    stage_sync<BLK>(); // X is read; Y overwrites the same LDS
     if (live && role2)
     {
       const int o = r2a * YA + r2q * YQ;
@@ -358,7 +474,7 @@ __global__ void __launch_bounds__(block_threads<Q>())
         }
       }
     }
-    __syncthreads();
+    stage_sync<BLK>();
 
     // ---------------- F3 + quadrature-point work + B3 ---------------------
     T ipp[N], ipg[N], igp[N];
@@ -375,7 +491,7 @@ __global__ void __launch_bounds__(block_threads<Q>())
         }
       }
     }
-    __syncthreads(); // X and Y are idle from here until B3 writes Y
+    stage_sync<BLK>(); // X and Y are idle from here until B3 writes Y
     // first geometry half into LDS (its registers are freed before F3)
     if constexpr (GRAD)
     {
@@ -386,7 +502,7 @@ __global__ void __launch_bounds__(block_threads<Q>())
         if (p < GP0)
           Ls[p] = gp0[k];
       }
-      __syncthreads();
+      stage_sync<BLK>();
     }
 
     // F3: contract axis a
@@ -457,7 +573,7 @@ __global__ void __launch_bounds__(block_threads<Q>())
     }
     if constexpr (GRAD)
     {
-      __syncthreads();
+      stage_sync<BLK>();
 #pragma unroll
       for (int k = 0; k < GPT1; ++k)
       {
@@ -465,7 +581,7 @@ __global__ void __launch_bounds__(block_threads<Q>())
         if (p < GP1)
           Ls[p] = gp1[k];
       }
-      __syncthreads();
+      stage_sync<BLK>();
     }
     T ra_[N], rb_[N], rc_[N]; // B3 outputs, stored after the geometry is read
     if (live)
@@ -501,7 +617,7 @@ __global__ void __launch_bounds__(block_threads<Q>())
       }
     }
     if constexpr (GRAD)
-      __syncthreads(); // the geometry in LDS is consumed; Y may be rewritten
+      stage_sync<BLK>(); // the geometry in LDS is consumed; Y may be rewritten
     if (live)
     {
 #pragma unroll
@@ -516,7 +632,7 @@ __global__ void __launch_bounds__(block_threads<Q>())
           Yc[Y_RC * SY + o] = rc_[a];
       }
     }
-    __syncthreads();
+    stage_sync<BLK>();
 
     // ---------------- B2: contract axis c (transposed) --------------------
     T ra[Q], rb[Q], rc[Q];
@@ -534,7 +650,7 @@ __global__ void __launch_bounds__(block_threads<Q>())
           rc[q] = Yc[Y_RC * SY + o + q];
       }
     }
-    __syncthreads(); // Y is read; X overwrites the same LDS
+    stage_sync<BLK>(); // Y is read; X overwrites the same LDS
     if (live && role2)
     {
 #pragma unroll
@@ -561,7 +677,7 @@ __global__ void __launch_bounds__(block_threads<Q>())
           Xc[X_SB * SX + xo] = sb;
       }
     }
-    __syncthreads();
+    stage_sync<BLK>();
 
     // ---------------- B1: contract axis b (transposed) + store/scatter ----
     if (live && role1)
@@ -581,21 +697,59 @@ __global__ void __launch_bounds__(block_threads<Q>())
         load_bwd_row<T, N, Q, REF>(ph, b, rp);
         if constexpr (FULL)
           load_bwd_row<T, N, Q, REF>(gr, b, rg);
-        T acc = 0;
+        T w = 0;
 #pragma unroll
         for (int q = 0; q < Q; ++q)
         {
-          acc = fma(rp[q], s0[q], acc);
+          w = fma(rp[q], s0[q], w);
           if constexpr (FULL)
-            acc = fma(rg[q], sb[q], acc);
+            w = fma(rg[q], sb[q], w);
         }
-        if constexpr (GATHER)
-          atomic_add(y + idx[b], acc);
+        if constexpr (BLK)
+        {
+          const int s = slot0 + b * BZ;
+          if (gid[s] == idx[b])
+          {
+            atomicAdd(acc + s, w); // LDS
+            atomicAdd(cnt + s, 1);
+          }
+          else
+            atomic_add(y + idx[b], w);
+        }
+        else if constexpr (GATHER)
+          atomic_add(y + idx[b], w);
         else
-          y[cell * N3 + uoff + b * N] = acc;
+          y[cell * N3 + uoff + b * N] = w;
       }
     }
 
+    // blocked: write the box back.  A dof whose every touch came from this
+    // block is stored (or added without an atomic); the rest are atomics.
+    // Consecutive threads take consecutive slots, i.e. runs of consecutive
+    // global dofs for the structured dofmap.
+    if constexpr (BLK)
+    {
+      __syncthreads();
+      for (int s = threadIdx.x; s < BOX; s += blockDim.x)
+      {
+        const int c = cnt[s];
+        if (c > 0)
+        {
+          const int32_t g = gid[s];
+          const T v = acc[s];
+          const int t = ba.touch[g];
+          if (c == t && t < 255)
+          {
+            if (ba.overwrite)
+              y[g] = v;
+            else
+              y[g] += v;
+          }
+          else
+            atomic_add(y + g, v);
+        }
+      }
+    }
   }
 }
 
@@ -609,18 +763,85 @@ int launch_cells(const ApplyReq& r)
     return FEMTC_OK;
   if (nblk > 0x7fffffffLL)
     return FEMTC_ERR_INVALID_ARGUMENT;
-  hipLaunchKernelGGL((apply_cells<T, N, Q, OP, REF, GATHER>),
+  hipLaunchKernelGGL((apply_cells<T, N, Q, OP, REF, GATHER, 0>),
                      dim3((unsigned)nblk), dim3(block_threads<Q>()), 0,
                      r.stream, (const T*)r.tab, (const T*)r.x, (T*)r.y,
                      (const T*)r.G, (const T*)r.detJw, r.dofmap,
-                     (long long)r.num_cells, r.skip);
+                     (long long)r.num_cells, r.skip, BlockArgs{});
   return hip_status(hipGetLastError());
+}
+
+// Largest number of waves of a blocked-scatter workgroup.
+constexpr int BLOCKED_MAX_WAVES = 8;
+
+// Blocked scatter: one workgroup per cbx x cby x cbz block of cells.
+template <typename T, int N, int Q, int OP>
+int launch_blocked(const ApplyReq& r)
+{
+  if constexpr (Q * Q > 64)
+    return FEMTC_ERR_INVALID_ARGUMENT;
+  else
+  {
+    constexpr int CPB = cells_per_block<Q>();
+    constexpr int BW = BLOCKED_MAX_WAVES;
+    const int cells = r.cbx * r.cby * r.cbz;
+    const int waves = (cells + CPB - 1) / CPB;
+    if (r.cbx < 1 || r.cby < 1 || r.cbz < 1 || r.cbz % CPB != 0 ||
+        waves > BW || !r.touch ||
+        (long long)r.ncx * r.ncy * r.ncz != r.num_cells)
+      return FEMTC_ERR_INVALID_ARGUMENT;
+    if (r.num_cells == 0)
+      return FEMTC_OK;
+    BlockArgs ba{};
+    ba.ncx = r.ncx;
+    ba.ncy = r.ncy;
+    ba.ncz = r.ncz;
+    ba.cbx = r.cbx;
+    ba.cby = r.cby;
+    ba.cbz = r.cbz;
+    const long long nbx = (r.ncx + r.cbx - 1) / r.cbx;
+    ba.nby = (r.ncy + r.cby - 1) / r.cby;
+    ba.nbz = (r.ncz + r.cbz - 1) / r.cbz;
+    ba.touch = r.touch;
+    ba.overwrite = r.overwrite;
+    const long long nblk = nbx * ba.nby * ba.nbz;
+    if (nblk > 0x7fffffffLL)
+      return FEMTC_ERR_INVALID_ARGUMENT;
+    const size_t lds =
+        waves * blocked_work_bytes<T, N, Q, OP>() +
+        blocked_box_bytes<T>(blocked_box(N - 1, r.cbx, r.cby, r.cbz));
+    if (lds > 160 * 1024)
+      return FEMTC_ERR_INVALID_ARGUMENT;
+    auto kern = apply_cells<T, N, Q, OP, false, true, BW>;
+    if (lds > 64 * 1024 &&
+        hipFuncSetAttribute((const void*)kern,
+                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lds) != hipSuccess)
+      return FEMTC_ERR_HIP;
+    hipLaunchKernelGGL(kern, dim3((unsigned)nblk), dim3(64 * waves), lds,
+                       r.stream, (const T*)r.tab, (const T*)r.x, (T*)r.y,
+                       (const T*)r.G, (const T*)r.detJw, r.dofmap,
+                       (long long)r.num_cells, r.skip, ba);
+    return hip_status(hipGetLastError());
+  }
 }
 
 template <typename T, int N, int Q>
 int dispatch_op(const ApplyReq& r)
 {
   const bool g = r.gather != 0;
+  if (r.blocked)
+  {
+    if (!g || r.reference_mode)
+      return FEMTC_ERR_INVALID_ARGUMENT;
+    switch (r.op)
+    {
+    case OP_MASS: return launch_blocked<T, N, Q, OP_MASS>(r);
+    case OP_STIFF: return launch_blocked<T, N, Q, OP_STIFF>(r);
+    case OP_KPM: return launch_blocked<T, N, Q, OP_KPM>(r);
+    }
+    return FEMTC_ERR_INVALID_ARGUMENT;
+  }
   switch (r.op)
   {
   case OP_MASS:

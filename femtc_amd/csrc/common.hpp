@@ -40,6 +40,26 @@ struct ApplyReq
   // nothing (lets a solver queue iterations past convergence for free)
   const int* skip = nullptr;
   hipStream_t stream = nullptr;
+  // LDS-assembled scatter (gather only).  Cells are taken as numbered
+  // (ci*ncy + cj)*ncz + ck on an ncx x ncy x ncz grid and processed in
+  // cbx x cby x cbz blocks, one workgroup each; touch[g] = how many
+  // cell-dofs map to global dof g (saturated at 255).  Correct for any
+  // dofmap; fast when it is the structured one.
+  int blocked = 0;
+  int ncx = 0, ncy = 0, ncz = 0;
+  int cbx = 0, cby = 0, cbz = 0;
+  const uint8_t* touch = nullptr;
+  int overwrite = 0; // y was cleared: dofs owned by one block are stored
+};
+
+// Kernel-side view of the blocked-scatter parameters.
+struct BlockArgs
+{
+  int ncx, ncy, ncz; // mesh cells per axis
+  int cbx, cby, cbz; // cells per block per axis
+  int nby, nbz;      // blocks along the two fastest axes
+  const uint8_t* touch;
+  int overwrite;
 };
 
 using ApplyLauncher = int (*)(const ApplyReq&);

@@ -194,6 +194,40 @@ int femtc_geometry_affine(int64_t num_cells, int nq, const double* J,
                           void* G, void* detJw, int dtype, void* stream);
 
 /* ---------------------------------------------------------------------- */
+/* Scatter plans (no counterpart in the reference, which has no dofmap).   */
+/* A plan describes a dofmap whose cells are numbered (ci*ny + cj)*nz + ck */
+/* on an nx*ny*nz grid (as femtc_dofmap_structured numbers them).  Applies */
+/* through a plan process 3-D blocks of cells per workgroup, sum shared    */
+/* dofs in LDS and store the dofs only one block touches without atomics.  */
+/* Results are correct for any dofmap; the blocking pays off when the      */
+/* numbering is the structured one.  The plan keeps a pointer to dofmap    */
+/* (which must stay alive) and 1 byte per dof of touch counts.             */
+/* ---------------------------------------------------------------------- */
+typedef struct femtc_plan femtc_plan; /* opaque */
+int femtc_plan_create(femtc_plan** plan, int nx, int ny, int nz, int degree,
+                      const int32_t* dofmap, int64_t ndofs, void* stream);
+int femtc_plan_destroy(femtc_plan* plan);
+/* Block shape (cells per axis) for one (nq, dtype, op); 0,0,0 selects the
+ * direct atomic scatter.  cbz must be a multiple of 64/nq^2 and the block
+ * at most 8 waves of 64/nq^2 cells; nq <= 8. */
+int femtc_plan_set_block(femtc_plan* plan, int nq, int dtype, int op, int cbx,
+                         int cby, int cbz);
+int femtc_plan_get_block(const femtc_plan* plan, int nq, int dtype, int op,
+                         int* cbx, int* cby, int* cbz);
+/* y = A x (accumulate = 0: y overwritten) or y += A x (accumulate = 1). */
+int femtc_plan_apply(const femtc_plan* plan, int op, int dtype, int nq,
+                     const void* tables, const void* x, void* y,
+                     const void* G, const void* detJw, int accumulate,
+                     void* stream);
+/* Time every candidate block shape (and the direct scatter) on these
+ * buffers, reps applies each, and keep the fastest for (nq, dtype, op).
+ * y is left cleared.  best_us (may be NULL) receives its time per apply. */
+int femtc_plan_autotune(femtc_plan* plan, int op, int dtype, int nq,
+                        const void* tables, const void* x, void* y,
+                        const void* G, const void* detJw, int reps,
+                        void* stream, double* best_us);
+
+/* ---------------------------------------------------------------------- */
 /* Solver: Jacobi-preconditioned CG on the continuous operator, and FP64   */
 /* defect correction around an FP32 inner PCG.  The reference has no       */
 /* solver; these entry points have no counterpart there.                   */

@@ -124,3 +124,20 @@ def test_solver_create_rejects_bad_problems_without_touching_the_device():
     assert L.femtc_solver_create(None, C_.byref(p), F.F64, None) == \
         _lib.ERR_INVALID_ARGUMENT
     assert h.value is None
+
+
+def test_plan_create_validates_arguments_without_touching_the_device():
+    import ctypes as C_
+    L = F.lib()
+    h = C_.c_void_p()
+    assert L.femtc_plan_create(C_.byref(h), 0, 2, 2, 3, 1, 100, None) == \
+        _lib.ERR_INVALID_ARGUMENT
+    assert L.femtc_plan_create(C_.byref(h), 2, 2, 2, 9, 1, 100, None) == \
+        _lib.ERR_DEGREE_NOT_SUPPORTED
+    assert L.femtc_plan_create(C_.byref(h), 2, 2, 2, 3, None, 100, None) == \
+        _lib.ERR_INVALID_ARGUMENT
+    assert L.femtc_plan_create(None, 2, 2, 2, 3, 1, 100, None) == \
+        _lib.ERR_INVALID_ARGUMENT
+    assert h.value is None
+    assert L.femtc_plan_apply(None, 0, 0, 4, 1, 1, 1, None, 1, 0, None) == \
+        _lib.ERR_INVALID_ARGUMENT
