@@ -76,6 +76,15 @@ SIGNATURES = {
                                 _i32, _vp]),
     "femtc_plan_autotune": (_i32, [_vp, _i32, _i32, _i32, _vp, _vp, _vp, _vp,
                                    _vp, _i32, _vp, C.POINTER(C.c_double)]),
+    "femtc_batch_create": (_i32, [C.POINTER(_vp), _vp, _i32, _vp]),
+    "femtc_batch_run": (_i32, [_vp]),
+    "femtc_batch_graph_launch": (_i32, [_vp]),
+    "femtc_batch_autotune": (_i32, [_vp, _i32]),
+    "femtc_batch_stats": (_i32, [_vp, C.POINTER(_i32), C.POINTER(_i64),
+                                 C.POINTER(_i64)]),
+    "femtc_batch_group": (_i32, [_vp, _i32] + [C.POINTER(_i32)] * 7),
+    "femtc_batch_set_group": (_i32, [_vp, _i32, _i32]),
+    "femtc_batch_destroy": (_i32, [_vp]),
     "femtc_global_diagonal": (_i32, [_i32, _i32, _i32, _i32, _vp, _vp, _i64,
                                      _vp, _vp, _vp, _i64, _vp]),
     "femtc_solver_create": (_i32, [C.POINTER(_vp), _vp, _i32, _vp]),
@@ -91,6 +100,14 @@ class Problem(C.Structure):
     _fields_ = [("op", _i32), ("degree", _i32), ("nq", _i32),
                 ("num_cells", _i64), ("ndofs", _i64), ("dofmap", _vp),
                 ("tables", _vp), ("G", _vp), ("detJw", _vp)]
+
+
+class BatchItem(C.Structure):
+    """femtc_batch_item (include/femtc.h)."""
+    _fields_ = [("op", _i32), ("dtype", _i32), ("degree", _i32), ("nq", _i32),
+                ("tables", _vp), ("x", _vp), ("y", _vp), ("G", _vp),
+                ("detJw", _vp), ("dofmap", _vp), ("num_cells", _i64),
+                ("ndofs", _i64)]
 
 
 class SolveInfo(C.Structure):

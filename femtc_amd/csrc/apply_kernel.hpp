@@ -209,19 +209,17 @@ inline size_t blocked_box_bytes(size_t box)
   return box * (sizeof(T) + 2 * sizeof(int32_t));
 }
 
-// BW = 0: one workgroup per CPB consecutive cells, direct scatter.
+// The work of one workgroup (block index blk).
+// BW = 0: CPB consecutive cells, direct scatter.
 // BW > 0: blocked scatter, BW waves per workgroup and one block of
 // cbx x cby x cbz cells per workgroup (Q <= 8 and cbz % CPB == 0).
 template <typename T, int N, int Q, int OP, bool REF, bool GATHER, int BW>
-__global__ void __launch_bounds__(BW ? 64 * BW : block_threads<Q>())
-    apply_cells(const T* __restrict__ tab, const T* __restrict__ x,
-                T* __restrict__ y, const T* __restrict__ G,
-                const T* __restrict__ detJw, const int32_t* __restrict__ dofmap,
-                long long num_cells, const int* __restrict__ skip,
-                BlockArgs ba)
+__device__ __forceinline__ void
+apply_body(const T* __restrict__ tab, const T* __restrict__ x,
+           T* __restrict__ y, const T* __restrict__ G,
+           const T* __restrict__ detJw, const int32_t* __restrict__ dofmap,
+           long long num_cells, const BlockArgs& ba, long long blk)
 {
-  if (skip && *skip)
-    return;
   constexpr bool BLK = BW > 0;
   static_assert(!BLK || (GATHER && Q * Q <= 64), "blocked scatter layout");
   constexpr int CPB = cells_per_block<Q>();
@@ -302,7 +300,7 @@ __global__ void __launch_bounds__(BW ? 64 * BW : block_threads<Q>())
       acc[s] = T(0);
       cnt[s] = 0;
     }
-    const long long b = blockIdx.x;
+    const long long b = blk;
     const long long bk = b % ba.nbz;
     const long long bj = (b / ba.nbz) % ba.nby;
     const long long bi = b / ((long long)ba.nbz * ba.nby);
@@ -341,7 +339,7 @@ __global__ void __launch_bounds__(BW ? 64 * BW : block_threads<Q>())
     }
     else
     {
-      cell0 = (long long)blockIdx.x * CPB;
+      cell0 = blk * CPB;
       ncells_blk = num_cells - cell0 < CPB ? num_cells - cell0 : (long long)CPB;
     }
     const long long cell = cell0 + lc;
@@ -752,6 +750,119 @@ __global__ void __launch_bounds__(BW ? 64 * BW : block_threads<Q>())
     }
   }
 }
+
+template <typename T, int N, int Q, int OP, bool REF, bool GATHER, int BW>
+__global__ void __launch_bounds__(BW ? 64 * BW : block_threads<Q>())
+    apply_cells(const T* __restrict__ tab, const T* __restrict__ x,
+                T* __restrict__ y, const T* __restrict__ G,
+                const T* __restrict__ detJw, const int32_t* __restrict__ dofmap,
+                long long num_cells, const int* __restrict__ skip,
+                BlockArgs ba)
+{
+  if (skip && *skip)
+    return;
+  apply_body<T, N, Q, OP, REF, GATHER, BW>(tab, x, y, G, detJw, dofmap,
+                                           num_cells, ba, blockIdx.x);
+}
+
+// Batched launch: many independent problems of one (T, N, Q, OP, GATHER)
+// in one grid.  Problem i owns the workgroup-sized blocks
+// [blk_begin_i, blk_begin_{i+1}); each workgroup handles `per_wg`
+// consecutive blocks (a persistent-style knob chosen by the autotuner) and
+// finds each block's problem by binary search over blk_begin.
+template <typename T, int N, int Q, int OP, bool GATHER>
+__global__ void __launch_bounds__(block_threads<Q>())
+    apply_batch(const BatchDesc* __restrict__ items, int nitems,
+                long long nblk_total, int per_wg)
+{
+  // the problems' first blocks, searched in LDS (one global round trip
+  // instead of one per bisection step)
+  __shared__ long long first_blk[BATCH_MAX_ITEMS];
+  for (int i = threadIdx.x; i < nitems; i += blockDim.x)
+    first_blk[i] = items[i].blk_begin;
+  __syncthreads();
+  for (int it = 0; it < per_wg; ++it)
+  {
+    const long long b = (long long)blockIdx.x * per_wg + it;
+    if (b >= nblk_total)
+      return; // uniform over the workgroup
+    int lo = 0, hi = nitems - 1;
+    while (lo < hi)
+    {
+      const int mid = (lo + hi + 1) >> 1;
+      if (first_blk[mid] <= b)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    const BatchDesc d = items[lo];
+    apply_body<T, N, Q, OP, false, GATHER, 0>(
+        (const T*)d.tab, (const T*)d.x, (T*)d.y, (const T*)d.G,
+        (const T*)d.detJw, d.dofmap, d.num_cells, BlockArgs{},
+        b - d.blk_begin);
+    __syncthreads(); // the next block reuses the LDS work area
+  }
+}
+
+template <typename T, int N, int Q, int OP, bool GATHER>
+int launch_batch_t(const BatchReq& r)
+{
+  if (r.nblk_total == 0)
+    return FEMTC_OK;
+  const int per = r.per_wg < 1 ? 1 : r.per_wg;
+  const long long grid = (r.nblk_total + per - 1) / per;
+  if (grid > 0x7fffffffLL)
+    return FEMTC_ERR_INVALID_ARGUMENT;
+  hipLaunchKernelGGL((apply_batch<T, N, Q, OP, GATHER>), dim3((unsigned)grid),
+                     dim3(block_threads<Q>()), 0, r.stream, r.items, r.nitems,
+                     r.nblk_total, per);
+  return hip_status(hipGetLastError());
+}
+
+template <typename T, int N, int Q, int OP>
+int launch_batch_g(const BatchReq& r)
+{
+  return r.gather ? launch_batch_t<T, N, Q, OP, true>(r)
+                  : launch_batch_t<T, N, Q, OP, false>(r);
+}
+
+template <int N, int Q>
+int launch_batch_nq(const BatchReq& r)
+{
+  const bool d = r.dtype == FEMTC_F64;
+  if (r.dtype != FEMTC_F64 && r.dtype != FEMTC_F32)
+    return FEMTC_ERR_INVALID_ARGUMENT;
+  switch (r.op)
+  {
+  case OP_MASS:
+    return d ? launch_batch_g<double, N, Q, OP_MASS>(r)
+             : launch_batch_g<float, N, Q, OP_MASS>(r);
+  case OP_STIFF:
+    return d ? launch_batch_g<double, N, Q, OP_STIFF>(r)
+             : launch_batch_g<float, N, Q, OP_STIFF>(r);
+  case OP_KPM:
+    return d ? launch_batch_g<double, N, Q, OP_KPM>(r)
+             : launch_batch_g<float, N, Q, OP_KPM>(r);
+  }
+  return FEMTC_ERR_INVALID_ARGUMENT;
+}
+
+template <int N>
+int launch_batch_n(const BatchReq& r)
+{
+  if (r.N != N)
+    return FEMTC_ERR_DEGREE_NOT_SUPPORTED;
+  switch (r.Q - N)
+  {
+  case 0: return launch_batch_nq<N, N>(r);
+  case 1: return launch_batch_nq<N, N + 1>(r);
+  case 2: return launch_batch_nq<N, N + 2>(r);
+  }
+  return FEMTC_ERR_INVALID_ARGUMENT;
+}
+
+// cells per workgroup of the direct kernel, for a runtime nq
+inline int cells_per_workgroup(int nq) { return nq * nq <= 64 ? 64 / (nq * nq) : 1; }
 
 // Launch one instance: one workgroup per batch of CPB cells.
 template <typename T, int N, int Q, int OP, bool REF, bool GATHER>

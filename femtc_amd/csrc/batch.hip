@@ -1,0 +1,511 @@
+// Batched operator applies: many independent problems (meshes) of uneven
+// size, degree, operator and precision in a few launches.  Problems that
+// share (op, dtype, degree, nq, gather) form a group and run as ONE launch
+// of apply_batch (each workgroup looks its problem up by block index); the
+// groups run on up to four forked streams so small groups overlap; the whole
+// sequence can be captured once into a hipGraph and replayed.  The
+// per-group knob (blocks per workgroup) is chosen by femtc_batch_autotune.
+#include <algorithm>
+#include <map>
+#include <tuple>
+#include <vector>
+
+#include "common.hpp"
+
+namespace femtc
+{
+namespace
+{
+
+struct FillDesc
+{
+  void* p;
+  long long n;
+  int dtype;
+};
+
+__global__ void __launch_bounds__(256)
+    k_fill_batch(const FillDesc* __restrict__ d)
+{
+  const FillDesc f = d[blockIdx.y];
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < f.n;
+       i += (long long)gridDim.x * 256)
+  {
+    if (f.dtype == FEMTC_F64)
+      ((double*)f.p)[i] = 0.0;
+    else
+      ((float*)f.p)[i] = 0.0f;
+  }
+}
+
+BatchLauncher batch_launcher(int N)
+{
+  switch (N)
+  {
+  case 2: return launch_batch_n2;
+  case 3: return launch_batch_n3;
+  case 4: return launch_batch_n4;
+  case 5: return launch_batch_n5;
+  case 6: return launch_batch_n6;
+  case 7: return launch_batch_n7;
+  case 8: return launch_batch_n8;
+  case 9: return launch_batch_n9;
+  }
+  return nullptr;
+}
+
+int cells_per_wg(int nq) { return nq * nq <= 64 ? 64 / (nq * nq) : 1; }
+
+constexpr int NSTREAMS = 4;
+
+struct Group
+{
+  BatchReq req;
+  size_t first = 0; // offset of its descriptors in the device array
+  std::vector<BatchDesc> host;
+  long long cells = 0;
+  // 0: one fused apply_batch launch; 1: one launch per problem
+  int per_problem = 1;
+};
+
+// Launch a group: fused, or one direct launch per problem.
+int launch_group(const Group& g, hipStream_t s)
+{
+  if (!g.per_problem)
+  {
+    BatchReq r = g.req;
+    r.stream = s;
+    BatchLauncher f = batch_launcher(r.N);
+    return f ? f(r) : FEMTC_ERR_DEGREE_NOT_SUPPORTED;
+  }
+  for (const BatchDesc& d : g.host)
+  {
+    ApplyReq a;
+    a.op = g.req.op;
+    a.dtype = g.req.dtype;
+    a.N = g.req.N;
+    a.Q = g.req.Q;
+    a.gather = g.req.gather;
+    a.tab = d.tab;
+    a.x = d.x;
+    a.y = d.y;
+    a.G = d.G;
+    a.detJw = d.detJw;
+    a.dofmap = d.dofmap;
+    a.num_cells = d.num_cells;
+    a.stream = s;
+    const int st = run(a);
+    if (st != FEMTC_OK)
+      return st;
+  }
+  return FEMTC_OK;
+}
+
+} // namespace
+} // namespace femtc
+
+using namespace femtc;
+
+struct femtc_batch
+{
+  std::vector<Group> groups;
+  BatchDesc* d_desc = nullptr;
+  FillDesc* d_fill = nullptr;
+  int nfill = 0;
+  long long max_fill = 0;
+  hipStream_t stream = nullptr;           // the caller's stream
+  hipStream_t side[NSTREAMS] = {};        // forked streams
+  hipEvent_t ev_fork = nullptr;
+  hipEvent_t ev_join[NSTREAMS] = {};
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int64_t total_cells = 0;
+  int64_t total_dofs = 0;
+};
+
+namespace
+{
+
+void release(femtc_batch* b)
+{
+  if (!b)
+    return;
+  if (b->exec)
+    (void)hipGraphExecDestroy(b->exec);
+  if (b->graph)
+    (void)hipGraphDestroy(b->graph);
+  for (int i = 0; i < NSTREAMS; ++i)
+  {
+    if (b->side[i])
+      (void)hipStreamDestroy(b->side[i]);
+    if (b->ev_join[i])
+      (void)hipEventDestroy(b->ev_join[i]);
+  }
+  if (b->ev_fork)
+    (void)hipEventDestroy(b->ev_fork);
+  if (b->d_desc)
+    (void)hipFree(b->d_desc);
+  if (b->d_fill)
+    (void)hipFree(b->d_fill);
+  delete b;
+}
+
+int check_item(const femtc_batch_item& it, int i)
+{
+  const std::string where = "batch item " + std::to_string(i) + ": ";
+  int st = check_shape(it.degree, it.nq, FEMTC_MODE_CORRECT);
+  if (st != FEMTC_OK)
+    return st;
+  if ((st = check_dtype(it.dtype)) != FEMTC_OK)
+    return st;
+  if (it.op < FEMTC_OP_MASS || it.op > FEMTC_OP_KPM)
+    return fail(FEMTC_ERR_INVALID_ARGUMENT, where + "op must be a femtc_op");
+  if (it.num_cells < 0 || !it.tables || !it.x || !it.y)
+    return fail(FEMTC_ERR_INVALID_ARGUMENT, where + "missing buffer");
+  if (it.op != FEMTC_OP_MASS && !it.G)
+    return fail(FEMTC_ERR_INVALID_ARGUMENT, where + "G is required");
+  if (it.op != FEMTC_OP_STIFFNESS && !it.detJw)
+    return fail(FEMTC_ERR_INVALID_ARGUMENT, where + "detJw is required");
+  const uintptr_t galign = 2 * (it.dtype == FEMTC_F64 ? 8 : 4);
+  if (it.G && (uintptr_t)it.G % galign != 0)
+    return fail(FEMTC_ERR_INVALID_ARGUMENT,
+                where + "G must be aligned to two elements");
+  if (it.dofmap && it.ndofs < 1)
+    return fail(FEMTC_ERR_INVALID_ARGUMENT,
+                where + "a continuous item needs ndofs >= 1");
+  return FEMTC_OK;
+}
+
+// Launch every group once (fill of continuous outputs first), forking the
+// groups over the side streams and joining back into b->stream.
+int run_all(femtc_batch* b, hipStream_t s)
+{
+  if (b->nfill > 0)
+  {
+    const long long blocks = std::min<long long>(
+        std::max<long long>((b->max_fill + 255) / 256, 1), 1024);
+    hipLaunchKernelGGL(k_fill_batch, dim3((unsigned)blocks, b->nfill),
+                       dim3(256), 0, s, (const FillDesc*)b->d_fill);
+  }
+  if (hipEventRecord(b->ev_fork, s) != hipSuccess)
+    return fail(FEMTC_ERR_HIP, "batch: event record failed");
+  for (int i = 0; i < NSTREAMS; ++i)
+    if (hipStreamWaitEvent(b->side[i], b->ev_fork, 0) != hipSuccess)
+      return fail(FEMTC_ERR_HIP, "batch: stream wait failed");
+  for (size_t g = 0; g < b->groups.size(); ++g)
+  {
+    const int st = launch_group(b->groups[g], b->side[g % NSTREAMS]);
+    if (st != FEMTC_OK)
+      return fail(st, std::string("batch launch: ") +
+                          hipGetErrorString(hipGetLastError()));
+  }
+  for (int i = 0; i < NSTREAMS; ++i)
+  {
+    if (hipEventRecord(b->ev_join[i], b->side[i]) != hipSuccess ||
+        hipStreamWaitEvent(s, b->ev_join[i], 0) != hipSuccess)
+      return fail(FEMTC_ERR_HIP, "batch: join failed");
+  }
+  return FEMTC_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int femtc_batch_create(femtc_batch** out, const femtc_batch_item* items,
+                       int nitems, void* stream)
+{
+  if (!out || (nitems > 0 && !items) || nitems < 0)
+    return fail(FEMTC_ERR_INVALID_ARGUMENT, "bad batch arguments");
+  *out = nullptr;
+  for (int i = 0; i < nitems; ++i)
+  {
+    const int st = check_item(items[i], i);
+    if (st != FEMTC_OK)
+      return st;
+  }
+  femtc_batch* b = new femtc_batch;
+  b->stream = (hipStream_t)stream;
+  // group by (op, dtype, N, Q, gather)
+  std::map<std::tuple<int, int, int, int, int>, size_t> index;
+  std::vector<FillDesc> fills;
+  for (int i = 0; i < nitems; ++i)
+  {
+    const femtc_batch_item& it = items[i];
+    const int N = it.degree + 1, gather = it.dofmap ? 1 : 0;
+    auto key = std::make_tuple(it.op, it.dtype, N, it.nq, gather);
+    auto f = index.find(key);
+    if (f == index.end() ||
+        b->groups[f->second].host.size() >= (size_t)BATCH_MAX_ITEMS)
+    {
+      Group g;
+      g.req.op = it.op;
+      g.req.dtype = it.dtype;
+      g.req.N = N;
+      g.req.Q = it.nq;
+      g.req.gather = gather;
+      index[key] = b->groups.size();
+      b->groups.push_back(g);
+      f = index.find(key);
+    }
+    Group& g = b->groups[f->second];
+    const int cpw = cells_per_wg(it.nq);
+    BatchDesc d;
+    d.tab = it.tables;
+    d.x = it.x;
+    d.y = it.y;
+    d.G = it.G;
+    d.detJw = it.detJw;
+    d.dofmap = it.dofmap;
+    d.num_cells = it.num_cells;
+    d.blk_begin = g.req.nblk_total;
+    if (it.num_cells > 0)
+    {
+      g.host.push_back(d);
+      g.req.nblk_total += (it.num_cells + cpw - 1) / cpw;
+      g.cells += it.num_cells;
+    }
+    b->total_cells += it.num_cells;
+    if (it.dofmap)
+    {
+      fills.push_back(FillDesc{it.y, (long long)it.ndofs, it.dtype});
+      b->max_fill = std::max<long long>(b->max_fill, it.ndofs);
+      b->total_dofs += it.ndofs;
+    }
+    else
+      b->total_dofs += it.num_cells * (int64_t)N * N * N;
+  }
+  // drop empty groups; largest groups first
+  b->groups.erase(std::remove_if(b->groups.begin(), b->groups.end(),
+                                 [](const Group& g) { return g.host.empty(); }),
+                  b->groups.end());
+  std::stable_sort(b->groups.begin(), b->groups.end(),
+                   [](const Group& a, const Group& c) {
+                     return a.req.nblk_total > c.req.nblk_total;
+                   });
+  size_t ndesc = 0;
+  for (auto& g : b->groups)
+  {
+    g.first = ndesc;
+    ndesc += g.host.size();
+  }
+  int st = FEMTC_OK;
+  if ((ndesc && hipMalloc((void**)&b->d_desc, ndesc * sizeof(BatchDesc)) !=
+                    hipSuccess) ||
+      (!fills.empty() &&
+       hipMalloc((void**)&b->d_fill, fills.size() * sizeof(FillDesc)) !=
+           hipSuccess))
+    st = fail(FEMTC_ERR_OUT_OF_MEMORY, "batch descriptor allocation failed");
+  for (auto& g : b->groups)
+  {
+    if (st != FEMTC_OK)
+      break;
+    g.req.items = b->d_desc + g.first;
+    g.req.nitems = (int)g.host.size();
+    if (hipMemcpy(b->d_desc + g.first, g.host.data(),
+                  g.host.size() * sizeof(BatchDesc),
+                  hipMemcpyHostToDevice) != hipSuccess)
+      st = fail(FEMTC_ERR_HIP, "batch descriptor upload failed");
+  }
+  if (st == FEMTC_OK && !fills.empty() &&
+      hipMemcpy(b->d_fill, fills.data(), fills.size() * sizeof(FillDesc),
+                hipMemcpyHostToDevice) != hipSuccess)
+    st = fail(FEMTC_ERR_HIP, "batch descriptor upload failed");
+  b->nfill = (int)fills.size();
+  if (b->nfill > 65535)
+    st = fail(FEMTC_ERR_INVALID_ARGUMENT, "at most 65535 continuous items");
+  for (int i = 0; i < NSTREAMS && st == FEMTC_OK; ++i)
+    if (hipStreamCreateWithFlags(&b->side[i], hipStreamNonBlocking) !=
+            hipSuccess ||
+        hipEventCreateWithFlags(&b->ev_join[i], hipEventDisableTiming) !=
+            hipSuccess)
+      st = fail(FEMTC_ERR_HIP, "batch stream creation failed");
+  if (st == FEMTC_OK &&
+      hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) !=
+          hipSuccess)
+    st = fail(FEMTC_ERR_HIP, "batch event creation failed");
+  if (st != FEMTC_OK)
+  {
+    release(b);
+    return st;
+  }
+  *out = b;
+  return FEMTC_OK;
+}
+
+int femtc_batch_run(femtc_batch* b)
+{
+  if (!b)
+    return fail(FEMTC_ERR_INVALID_ARGUMENT, "batch is NULL");
+  return run_all(b, b->stream);
+}
+
+int femtc_batch_graph_launch(femtc_batch* b)
+{
+  if (!b)
+    return fail(FEMTC_ERR_INVALID_ARGUMENT, "batch is NULL");
+  if (!b->exec)
+  {
+    // capture on a private stream (the legacy default stream cannot be
+    // captured), then replay on the caller's stream
+    hipStream_t cs = nullptr;
+    if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess)
+      return fail(FEMTC_ERR_HIP, "graph: stream creation failed");
+    int st = FEMTC_OK;
+    if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) !=
+        hipSuccess)
+      st = fail(FEMTC_ERR_HIP, "graph: begin capture failed");
+    if (st == FEMTC_OK)
+    {
+      const int rs = run_all(b, cs);
+      hipGraph_t g = nullptr;
+      const hipError_t e = hipStreamEndCapture(cs, &g);
+      if (rs != FEMTC_OK)
+        st = rs;
+      else if (e != hipSuccess)
+        st = fail(FEMTC_ERR_HIP, std::string("graph: end capture failed: ") +
+                                     hipGetErrorString(e));
+      else
+        b->graph = g;
+      if (st != FEMTC_OK && g)
+        (void)hipGraphDestroy(g);
+    }
+    (void)hipStreamDestroy(cs);
+    if (st != FEMTC_OK)
+      return st;
+    if (hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0) !=
+        hipSuccess)
+    {
+      b->exec = nullptr;
+      return fail(FEMTC_ERR_HIP, "graph: instantiate failed");
+    }
+  }
+  if (hipGraphLaunch(b->exec, b->stream) != hipSuccess)
+    return fail(FEMTC_ERR_HIP, std::string("graph launch failed: ") +
+                                   hipGetErrorString(hipGetLastError()));
+  return FEMTC_OK;
+}
+
+int femtc_batch_autotune(femtc_batch* b, int reps)
+{
+  if (!b)
+    return fail(FEMTC_ERR_INVALID_ARGUMENT, "batch is NULL");
+  if (reps < 1)
+    reps = 3;
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+    return fail(FEMTC_ERR_HIP, "event creation failed");
+  int st = FEMTC_OK;
+  for (auto& g : b->groups)
+  {
+    double best = 1e300;
+    int bestk = 1, best_pp = 1;
+    // k = 0: one launch per problem; k > 0: fused, k blocks per workgroup
+    for (int k : {0, 1, 2, 4, 8})
+    {
+      Group t = g;
+      t.per_problem = k == 0;
+      t.req.per_wg = k > 0 ? k : 1;
+      st = launch_group(t, b->stream); // warm-up
+      (void)hipEventRecord(e0, b->stream);
+      for (int i = 0; i < reps && st == FEMTC_OK; ++i)
+        st = launch_group(t, b->stream);
+      (void)hipEventRecord(e1, b->stream);
+      if (st != FEMTC_OK || hipEventSynchronize(e1) != hipSuccess)
+        break;
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      if (ms < best)
+      {
+        best = ms;
+        bestk = t.req.per_wg;
+        best_pp = t.per_problem;
+      }
+    }
+    if (st != FEMTC_OK)
+      break;
+    g.req.per_wg = bestk;
+    g.per_problem = best_pp;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (st != FEMTC_OK)
+    return fail(st, "batch autotune failed");
+  // a captured graph holds the old launch shapes
+  if (b->exec)
+  {
+    (void)hipGraphExecDestroy(b->exec);
+    b->exec = nullptr;
+  }
+  if (b->graph)
+  {
+    (void)hipGraphDestroy(b->graph);
+    b->graph = nullptr;
+  }
+  return FEMTC_OK;
+}
+
+int femtc_batch_stats(const femtc_batch* b, int* ngroups, int64_t* cells,
+                      int64_t* dofs)
+{
+  if (!b)
+    return fail(FEMTC_ERR_INVALID_ARGUMENT, "batch is NULL");
+  if (ngroups)
+    *ngroups = (int)b->groups.size();
+  if (cells)
+    *cells = b->total_cells;
+  if (dofs)
+    *dofs = b->total_dofs;
+  return FEMTC_OK;
+}
+
+int femtc_batch_group(const femtc_batch* b, int g, int* op, int* dtype,
+                      int* degree, int* nq, int* gather, int* nitems,
+                      int* per_wg)
+{
+  if (!b || g < 0 || g >= (int)b->groups.size())
+    return fail(FEMTC_ERR_INVALID_ARGUMENT, "no such group");
+  const BatchReq& r = b->groups[g].req;
+  if (op)
+    *op = r.op;
+  if (dtype)
+    *dtype = r.dtype;
+  if (degree)
+    *degree = r.N - 1;
+  if (nq)
+    *nq = r.Q;
+  if (gather)
+    *gather = r.gather;
+  if (nitems)
+    *nitems = r.nitems;
+  if (per_wg)
+    *per_wg = b->groups[g].per_problem ? 0 : r.per_wg;
+  return FEMTC_OK;
+}
+
+int femtc_batch_set_group(femtc_batch* b, int g, int per_wg)
+{
+  if (!b || g < 0 || g >= (int)b->groups.size() || per_wg < 0 || per_wg > 64)
+    return fail(FEMTC_ERR_INVALID_ARGUMENT, "no such group or bad per_wg");
+  b->groups[g].per_problem = per_wg == 0;
+  b->groups[g].req.per_wg = per_wg > 0 ? per_wg : 1;
+  if (b->exec)
+  {
+    (void)hipGraphExecDestroy(b->exec);
+    b->exec = nullptr;
+  }
+  if (b->graph)
+  {
+    (void)hipGraphDestroy(b->graph);
+    b->graph = nullptr;
+  }
+  return FEMTC_OK;
+}
+
+int femtc_batch_destroy(femtc_batch* b)
+{
+  release(b);
+  return FEMTC_OK;
+}
+
+} // extern "C"

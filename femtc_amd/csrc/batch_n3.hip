@@ -1,0 +1,8 @@
+// Batched-launch instances for N = 3 dofs per direction (degree 2),
+// Q in {N, N+1, N+2}, FP64 and FP32, cell-local and gathered.
+#include "apply_kernel.hpp"
+
+namespace femtc
+{
+int launch_batch_n3(const BatchReq& r) { return launch_batch_n<3>(r); }
+} // namespace femtc

@@ -1,0 +1,8 @@
+// Batched-launch instances for N = 7 dofs per direction (degree 6),
+// Q in {N, N+1, N+2}, FP64 and FP32, cell-local and gathered.
+#include "apply_kernel.hpp"
+
+namespace femtc
+{
+int launch_batch_n7(const BatchReq& r) { return launch_batch_n<7>(r); }
+} // namespace femtc

@@ -52,6 +52,46 @@ struct ApplyReq
   int overwrite = 0; // y was cleared: dofs owned by one block are stored
 };
 
+// One problem of a batched launch (device-resident descriptor).
+struct BatchDesc
+{
+  const void* tab;
+  const void* x;
+  void* y;
+  const void* G;
+  const void* detJw;
+  const int32_t* dofmap; // NULL for cell-local problems
+  long long num_cells;
+  long long blk_begin; // first workgroup-sized block of this problem
+};
+
+// Problems per batched launch (their first blocks are searched in LDS).
+constexpr int BATCH_MAX_ITEMS = 64;
+
+// One batched launch: every problem shares (op, dtype, N, Q, gather).
+struct BatchReq
+{
+  int op = OP_MASS;
+  int dtype = FEMTC_F64;
+  int N = 0, Q = 0;
+  int gather = 0;
+  const BatchDesc* items = nullptr; // device array, sorted by blk_begin
+  int nitems = 0;
+  long long nblk_total = 0;
+  int per_wg = 1;
+  hipStream_t stream = nullptr;
+};
+
+using BatchLauncher = int (*)(const BatchReq&);
+int launch_batch_n2(const BatchReq&);
+int launch_batch_n3(const BatchReq&);
+int launch_batch_n4(const BatchReq&);
+int launch_batch_n5(const BatchReq&);
+int launch_batch_n6(const BatchReq&);
+int launch_batch_n7(const BatchReq&);
+int launch_batch_n8(const BatchReq&);
+int launch_batch_n9(const BatchReq&);
+
 // Kernel-side view of the blocked-scatter parameters.
 struct BlockArgs
 {

@@ -228,6 +228,56 @@ int femtc_plan_autotune(femtc_plan* plan, int op, int dtype, int nq,
                         void* stream, double* best_us);
 
 /* ---------------------------------------------------------------------- */
+/* Batches (no counterpart in the reference): many independent problems    */
+/* of uneven size, degree, operator and precision applied together.        */
+/* Problems sharing (op, dtype, degree, nq, cell-local/continuous) form a  */
+/* group (at most 64 problems) that runs either as one fused launch        */
+/* (each workgroup finds its problem by block index) or as one launch per  */
+/* problem, whichever femtc_batch_autotune measured faster (default: per   */
+/* problem).  Groups are spread over four forked streams; the sequence can */
+/* be captured once into a hipGraph and replayed.  Every output is         */
+/* overwritten (continuous y is cleared first).  The batch keeps the item  */
+/* pointers: the buffers must stay alive and in place.                     */
+/* ---------------------------------------------------------------------- */
+typedef struct femtc_batch_item
+{
+  int op;     /* femtc_op */
+  int dtype;  /* FEMTC_F64 or FEMTC_F32: every array of the item */
+  int degree;
+  int nq;
+  const void* tables; /* [phi | dphi] */
+  const void* x;      /* continuous: [ndofs]; cell-local: [num_cells][N^3] */
+  void* y;            /* same shape as x */
+  const void* G;      /* [num_cells][nq^3][6] or NULL (mass) */
+  const void* detJw;  /* [num_cells][nq^3] or NULL (stiffness) */
+  const int32_t* dofmap; /* [num_cells][N^3], or NULL for cell-local */
+  int64_t num_cells;
+  int64_t ndofs; /* continuous items: length of x and y */
+} femtc_batch_item;
+
+typedef struct femtc_batch femtc_batch; /* opaque */
+int femtc_batch_create(femtc_batch** batch, const femtc_batch_item* items,
+                       int nitems, void* stream);
+/* Apply every item, stream-ordered on the batch's stream. */
+int femtc_batch_run(femtc_batch* batch);
+/* The same through a hipGraph: captured at the first call (and after an
+ * autotune), replayed afterwards. */
+int femtc_batch_graph_launch(femtc_batch* batch);
+/* Time, for every group, one launch per problem and the fused launch with
+ * 1, 2, 4 and 8 blocks per workgroup (reps launches each) and keep the
+ * fastest; synchronous; outputs are left undefined. */
+int femtc_batch_autotune(femtc_batch* batch, int reps);
+int femtc_batch_stats(const femtc_batch* batch, int* ngroups,
+                      int64_t* total_cells, int64_t* total_dofs);
+/* per_wg = 0: the group runs one launch per problem; per_wg = k > 0: one
+ * fused launch, k blocks per workgroup.  set_group drops a captured graph. */
+int femtc_batch_set_group(femtc_batch* batch, int group, int per_wg);
+int femtc_batch_group(const femtc_batch* batch, int group, int* op,
+                      int* dtype, int* degree, int* nq, int* gather,
+                      int* nitems, int* per_wg);
+int femtc_batch_destroy(femtc_batch* batch);
+
+/* ---------------------------------------------------------------------- */
 /* Solver: Jacobi-preconditioned CG on the continuous operator, and FP64   */
 /* defect correction around an FP32 inner PCG.  The reference has no       */
 /* solver; these entry points have no counterpart there.                   */

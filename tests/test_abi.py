@@ -141,3 +141,37 @@ def test_plan_create_validates_arguments_without_touching_the_device():
     assert h.value is None
     assert L.femtc_plan_apply(None, 0, 0, 4, 1, 1, 1, None, 1, 0, None) == \
         _lib.ERR_INVALID_ARGUMENT
+
+
+def test_batch_create_rejects_bad_items_without_touching_the_device():
+    import ctypes as C_
+    L = F.lib()
+    h = C_.c_void_p()
+    items = (_lib.BatchItem * 2)()
+    items[0] = _lib.BatchItem(F.OP_MASS, F.F64, 3, 4, 1, 1, 1, None, 1, None, 10, 0)
+    items[1] = _lib.BatchItem(F.OP_STIFFNESS, F.F64, 3, 4, 1, 1, 1, None, None,
+                              None, 10, 0)  # stiffness without G
+    assert L.femtc_batch_create(C_.byref(h), items, 2, None) == \
+        _lib.ERR_INVALID_ARGUMENT
+    assert "batch item 1" in _lib.last_error()
+    items[1] = _lib.BatchItem(F.OP_MASS, F.F64, 3, 9, 1, 1, 1, None, 1, None, 10, 0)
+    assert L.femtc_batch_create(C_.byref(h), items, 2, None) == \
+        _lib.ERR_INVALID_ARGUMENT  # nq > degree + 3
+    items[1] = _lib.BatchItem(F.OP_MASS, F.F64, 3, 4, 1, 1, 1, None, 1, 1, 10, 0)
+    assert L.femtc_batch_create(C_.byref(h), items, 2, None) == \
+        _lib.ERR_INVALID_ARGUMENT  # continuous without ndofs
+    assert h.value is None
+
+
+def test_uneven_batch_spec_is_seeded_and_clipped():
+    from femtc_amd import workloads as W
+    a = W.uneven_batch_spec(seed=1)
+    b = W.uneven_batch_spec(seed=1)
+    assert a == b and len(a) == 200
+    cells = [c for c, _, _ in a]
+    assert min(cells) >= 1000 and max(cells) <= 100000
+    assert {p for _, p, _ in a} <= set(range(2, 9))
+    assert sum(op == F.OP_MASS for _, _, op in a) == 100
+    nx, ny, nz = W.box_dims(12345)
+    assert abs(nx * ny * nz - 12345) / 12345 < 0.1
+    np.testing.assert_array_equal(W.uniform(5, 1000), O.uniform(5, 1000))
