@@ -13,10 +13,12 @@ from .operators import (fill_zero, global_apply, kpm_apply, mass_apply,
                         mass_operator, stiffness_apply, stiffness_operator)
 from .batch import Batch
 from .plan import Plan
+from .server import ResidentProblem, Server, latency_stats
 from .solver import PCGSolver, global_diagonal
 
 __all__ = [
-    "Batch", "Plan", "PCGSolver", "global_diagonal",
+    "Batch", "Plan", "PCGSolver", "ResidentProblem", "Server",
+    "global_diagonal", "latency_stats",
     "BF16", "F32", "F64", "MODE_CORRECT", "MODE_REFERENCE", "OP_KPM", "OP_MASS",
     "OP_STIFFNESS", "FemtcError", "lib", "Order", "string2order", "fill_zero",
     "basis_tables", "dofmap_structured", "gauss_rule", "geometry_affine",

@@ -85,6 +85,12 @@ SIGNATURES = {
     "femtc_batch_group": (_i32, [_vp, _i32] + [C.POINTER(_i32)] * 7),
     "femtc_batch_set_group": (_i32, [_vp, _i32, _i32]),
     "femtc_batch_destroy": (_i32, [_vp]),
+    "femtc_server_create": (_i32, [C.POINTER(_vp), _i32, _sz]),
+    "femtc_server_submit": (_i32, [_vp, _vp]),
+    "femtc_server_drain": (_i32, [_vp]),
+    "femtc_server_results": (_i32, [_vp, _vp, _i32, C.POINTER(_i32)]),
+    "femtc_server_memory": (_i32, [_vp, C.POINTER(_sz), C.POINTER(_i32)]),
+    "femtc_server_destroy": (_i32, [_vp]),
     "femtc_global_diagonal": (_i32, [_i32, _i32, _i32, _i32, _vp, _vp, _i64,
                                      _vp, _vp, _vp, _i64, _vp]),
     "femtc_solver_create": (_i32, [C.POINTER(_vp), _vp, _i32, _vp]),
@@ -108,6 +114,26 @@ class BatchItem(C.Structure):
                 ("tables", _vp), ("x", _vp), ("y", _vp), ("G", _vp),
                 ("detJw", _vp), ("dofmap", _vp), ("num_cells", _i64),
                 ("ndofs", _i64)]
+
+
+REQUEST_APPLY, REQUEST_SOLVE = 0, 1
+
+
+class Request(C.Structure):
+    """femtc_request (include/femtc.h)."""
+    _fields_ = [("kind", _i32), ("problem", _vp), ("input", _vp),
+                ("output", _vp), ("rtol", C.c_double),
+                ("inner_rtol", C.c_double), ("maxit", _i32),
+                ("inner_dtype", _i32), ("tag", _i64)]
+
+
+class RequestResult(C.Structure):
+    """femtc_request_result (include/femtc.h)."""
+    _fields_ = [("tag", _i64), ("kind", _i32), ("status", _i32),
+                ("slot", _i32), ("iterations", _i32),
+                ("outer_iterations", _i32), ("relres", C.c_double),
+                ("t_arrival", C.c_double), ("t_start", C.c_double),
+                ("t_end", C.c_double)]
 
 
 class SolveInfo(C.Structure):

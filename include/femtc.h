@@ -337,6 +337,61 @@ int femtc_solver_solve(femtc_solver* solver, const double* f, double* u,
 int femtc_solver_diagonal(const femtc_solver* solver, const double** diag);
 int femtc_solver_destroy(femtc_solver* solver);
 
+/* ---------------------------------------------------------------------- */
+/* Request server (no counterpart in the reference): apply and solve       */
+/* requests on resident problems, served by `slots` host threads with one  */
+/* HIP stream each, so requests admitted while others run overlap on the   */
+/* device.  Solvers stay resident per (problem, inner dtype, slot); their  */
+/* workspaces are admitted against `hbm_budget` bytes, idle ones evicted   */
+/* least-recently-used first; a request that does not fit yet waits while */
+/* later ones that fit go ahead.  Times are seconds since server creation. */
+/* ---------------------------------------------------------------------- */
+enum
+{
+  FEMTC_REQUEST_APPLY = 0, /* output = A input (FP64, overwritten) */
+  FEMTC_REQUEST_SOLVE = 1  /* output = A^-1 input to rtol */
+};
+
+typedef struct femtc_request
+{
+  int kind;
+  const femtc_problem* problem; /* must outlive the request */
+  const void* input;            /* FP64 device vector of ndofs */
+  void* output;                 /* FP64 device vector of ndofs */
+  double rtol;                  /* solve */
+  double inner_rtol;            /* solve with inner_dtype FEMTC_F32 */
+  int maxit;                    /* solve */
+  int inner_dtype;              /* solve: FEMTC_F64 or FEMTC_F32 */
+  int64_t tag;                  /* returned with the result */
+} femtc_request;
+
+typedef struct femtc_request_result
+{
+  int64_t tag;
+  int kind;
+  int status;
+  int slot;
+  int iterations;       /* solve: CG iterations */
+  int outer_iterations; /* solve: refinement steps */
+  double relres;        /* solve: FP64 relative residual */
+  double t_arrival, t_start, t_end;
+} femtc_request_result;
+
+typedef struct femtc_server femtc_server; /* opaque */
+int femtc_server_create(femtc_server** server, int slots, size_t hbm_budget);
+/* Queue a request (non-blocking); FEMTC_ERR_OUT_OF_MEMORY if its solve
+ * workspace alone exceeds the budget. */
+int femtc_server_submit(femtc_server* server, const femtc_request* request);
+/* Block until every submitted request has completed. */
+int femtc_server_drain(femtc_server* server);
+/* Move up to max completed results to out (completion order); *n = count. */
+int femtc_server_results(femtc_server* server, femtc_request_result* out,
+                         int max, int* n);
+int femtc_server_memory(const femtc_server* server, size_t* used,
+                        int* cached_solvers);
+/* Finish the queued requests, then stop the slots and free the solvers. */
+int femtc_server_destroy(femtc_server* server);
+
 #ifdef __cplusplus
 }
 #endif

@@ -175,3 +175,14 @@ def test_uneven_batch_spec_is_seeded_and_clipped():
     nx, ny, nz = W.box_dims(12345)
     assert abs(nx * ny * nz - 12345) / 12345 < 0.1
     np.testing.assert_array_equal(W.uniform(5, 1000), O.uniform(5, 1000))
+
+
+def test_latency_stats():
+    res = [{"t_arrival": 0.0, "t_end": 0.010 * (k + 1), "kind": k % 2}
+           for k in range(100)]
+    st = F.latency_stats(res)
+    assert st["n"] == 100
+    assert abs(st["p50_ms"] - 505.0) < 1.0
+    assert st["p99_ms"] > 985.0
+    assert abs(st["solves_per_s"] - 50.0) < 1e-9
+    assert F.latency_stats([]) == {"n": 0}
