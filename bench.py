@@ -1,28 +1,39 @@
 """Benchmark: FP64 stiffness operator apply y = K x through the dofmap.
 
-Workload: structured 48^3 hexahedral mesh (110592 cells), degree P=6 Lagrange
-on GLL nodes, Q=7 Gauss-Legendre points per direction, int32 dofmap with
-shared faces (289^3 = 24,137,569 global dofs), per-cell affine Jacobians
-J = I + 0.1 U(-1,1) with G = detJ w J^-1 J^-T per point in FP64, x ~ U[-1,1].
+Workload (the metric's): structured 48^3 hexahedral mesh (110592 cells),
+degree P=6 Lagrange on GLL nodes, Q=7 Gauss-Legendre points per direction,
+int32 dofmap with shared faces (289^3 = 24,137,569 global dofs), per-cell
+affine Jacobians J = I + 0.1 U(-1,1) with G = detJ w J^-1 J^-T per point in
+FP64, x ~ U[-1,1].
 
 One step = one continuous operator application: clear y (fill kernel), then
 the fused gather -> contraction chain -> geometry -> transposed chain ->
 atomic scatter kernel.  Inputs are resident in HBM before the timed region
-starts.
+starts.  `value` is global dofs per second of the timed steps (GDoF/s).
 
-Prints ONE JSON line (rank 0).  ``value`` is global dofs per second (GDoF/s).
-``roofline`` prices the fused apply kernel against the MI355X HBM peak using
-the algorithmic bytes (x + y + dofmap + G once each), with the kernel's own
-duration taken from HIP events on the stream it runs on.  ``cpu_baseline``
-times the reference's own stiffness kernel (src/stiffness.hpp:34-99, built
-with -Ofast from the reference sources; oracle/_ref) on the host cores over
-a bounded sample of cells of the same shape.
+Beside the metric line's required fields it reports, on the same GPU:
+  roofline   the apply kernel against the MI355X HBM peak with algorithmic
+             bytes (x + y + dofmap + G once each) over the kernel's own time
+             from HIP events on its stream; `traffic` = HBM bytes per launch
+             from the committed FETCH_SIZE / WRITE_SIZE counter profile
+  cpu_baseline  the reference's own stiffness kernel (src/stiffness.hpp:34-99)
+             built -Ofast from the reference sources (oracle/_ref) on the host
+             cores, on a bounded sample of cells of the same shape
+  solve      (K+M)u = f on the same mesh to 1e-10: FP64 Jacobi-PCG vs FP32
+             inner PCG with FP64 defect correction (solves/s, error)
+  config2    the P=4 64^3 stiffness apply (GDoF/s, roofline fraction)
+  batch      200 uneven meshes in one batched, graph-captured launch set
+  serving    a 500 req/s Poisson stream of applies and solves (p50/p99)
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--apply-only]
+                  [--no-cpu-baseline] [--copy-calibration MiB]
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import re
 import sys
 import threading
 import time
@@ -34,18 +45,24 @@ sys.path.insert(0, ROOT)
 
 METRIC = "GDoF/s stiffness apply P=6 FP64 vs BF16-MFMA+refine @1e-10; p99 solve ms"
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles",
+                               "pmc_apply_stiffness_p6_q7_48cube_f64.txt")
 
 
 def _uniform(seed, n, lo, hi):
-    """Counter-based SplitMix64 uniforms (same generator as the oracle)."""
-    g = np.uint64(0x9E3779B97F4A7C15)
-    with np.errstate(over="ignore"):
-        z = np.uint64(seed) + np.arange(1, n + 1, dtype=np.uint64) * g
-        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
-        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
-        z = z ^ (z >> np.uint64(31))
-    u = (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
-    return lo + (hi - lo) * u
+    """Counter-based SplitMix64 uniforms (the workloads' generator)."""
+    from femtc_amd.workloads import uniform
+    return uniform(seed, n, lo, hi)
+
+
+def measured_traffic():
+    """HBM bytes per apply launch from the committed counter profile."""
+    try:
+        text = open(TRAFFIC_PROFILE).read()
+    except OSError:
+        return None
+    m = re.search(r"traffic_bytes_per_launch\s*=\s*(\d+)", text)
+    return int(m.group(1)) if m else None
 
 
 def cpu_baseline(P: int, ncells_global: int, ndofs_global: int,
@@ -132,6 +149,141 @@ def cpu_baseline(P: int, ncells_global: int, ndofs_global: int,
     }
 
 
+def structured(torch, F, n, P, want_detJw=False):
+    """Resident FP64 problem on an n^3 mesh at degree P, Q = P+1."""
+    dev = torch.device("cuda:0")
+    Q, N, C = P + 1, P + 1, n ** 3
+    ng = (n * P + 1) ** 3
+    tab = torch.from_numpy(F.basis_tables(P, Q)).to(dev)
+    J = _uniform(3, 9 * C, -1, 1) * 0.1
+    J = (J.reshape(C, 3, 3) + np.eye(3)).ravel()
+    dJ = torch.from_numpy(J).to(dev)
+    G = torch.empty(C * Q ** 3 * 6, device=dev, dtype=torch.float64)
+    D = (torch.empty(C * Q ** 3, device=dev, dtype=torch.float64)
+         if want_detJw else None)
+    F.geometry_affine(dJ, Q, G=G, detJw=D)
+    dm = torch.empty(C * N ** 3, device=dev, dtype=torch.int32)
+    F.dofmap_structured(n, n, n, P, dm)
+    del dJ
+    return {"tab": tab, "G": G, "D": D, "dm": dm, "C": C, "ng": ng, "P": P,
+            "Q": Q, "N": N}
+
+
+def time_apply(torch, F, pr, steps, warmup, x, y):
+    """Per step: fill y, y += K x.  Returns (wall ms/step, kernel ms)."""
+    stream = torch.cuda.current_stream()
+
+    def apply():
+        F.global_apply(F.OP_STIFFNESS, pr["tab"], x, y, pr["dm"], pr["C"],
+                       pr["P"], pr["Q"], G=pr["G"], accumulate=True)
+
+    for _ in range(warmup):
+        F.fill_zero(y)
+        apply()
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True),
+           torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    t0 = time.perf_counter()
+    for i in range(steps):
+        F.fill_zero(y)
+        ev[i][0].record(stream)
+        apply()
+        ev[i][1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kern_ms = float(np.mean([s.elapsed_time(e) for s, e in ev]))
+    return wall * 1e3 / steps, kern_ms
+
+
+def alg_bytes_stiffness(pr):
+    return 8 * pr["ng"] * 2 + 4 * pr["C"] * pr["N"] ** 3 + 48 * pr["C"] * pr["Q"] ** 3
+
+
+def bench_solve(torch, F, n=48, P=6):
+    """(K+M)u = f to 1e-10: FP64 PCG vs FP32 inner + FP64 refinement."""
+    pr = structured(torch, F, n, P, want_detJw=True)
+    f = torch.from_numpy(_uniform(7, pr["ng"], -1, 1)).cuda()
+    out = {"workload": f"(K+M)u=f, {n}^3 mesh, P={P}, Q={P + 1}, f~U[-1,1], "
+                       "rtol 1e-10 (FP64 residual)"}
+    sols = {}
+    for name, inner in (("fp64_pcg", F.F64), ("fp32_inner_fp64_refine", F.F32)):
+        with F.PCGSolver(F.OP_KPM, pr["tab"], pr["dm"], pr["C"], P, P + 1,
+                         pr["ng"], G=pr["G"], detJw=pr["D"], inner=inner) as S:
+            u = torch.empty(pr["ng"], device="cuda", dtype=torch.float64)
+            S.solve(f, u, rtol=1e-10)  # warm-up
+            infos = [S.solve(f, u, rtol=1e-10) for _ in range(2)]
+        ms = float(np.mean([i["seconds"] for i in infos])) * 1e3
+        out[name] = {"ms_per_solve": ms, "solves_per_s": 1e3 / ms,
+                     "iterations": infos[-1]["inner_iterations"],
+                     "outer": infos[-1]["outer_iterations"],
+                     "relres": infos[-1]["relres"]}
+        sols[name] = u
+    u64, u32 = sols["fp64_pcg"], sols["fp32_inner_fp64_refine"]
+    out["fp32_inner_fp64_refine"]["error_vs_fp64_solution"] = float(
+        (u32 - u64).norm() / u64.norm())
+    out["speedup_refined_vs_fp64"] = (out["fp64_pcg"]["ms_per_solve"] /
+                                      out["fp32_inner_fp64_refine"]["ms_per_solve"])
+    out["bf16_mfma_inner"] = "not built"
+    return out
+
+
+def bench_config2(torch, F, steps=30, warmup=60):
+    pr = structured(torch, F, 64, 4)
+    x = torch.from_numpy(_uniform(4, pr["ng"], -1, 1)).cuda()
+    y = torch.empty(pr["ng"], device="cuda", dtype=torch.float64)
+    ms, kern = time_apply(torch, F, pr, steps, warmup, x, y)
+    ab = alg_bytes_stiffness(pr)
+    return {"workload": "stiffness apply y=Kx, 64^3 mesh, P=4 GLL, Q=5, "
+                        "int32 dofmap, FP64",
+            "gdofs": pr["ng"] / (ms * 1e-3) / 1e9, "kernel_ms": kern,
+            "algorithmic_bytes": ab,
+            "roofline_frac": ab / (kern * 1e-3) / 1e9 / HBM_PEAK_GBS}
+
+
+def bench_batch(torch, F):
+    from femtc_amd import workloads as Wk
+    out = {}
+    for dname, dt, es in (("f64", torch.float64, 8), ("f32", torch.float32, 4)):
+        spec = Wk.uneven_batch_spec(seed=0)
+        items = [Wk.structured_problem(torch, Wk.box_dims(c), P, P + 1, op, dt,
+                                       1000 + i) for i, (c, P, op) in enumerate(spec)]
+        nbytes = sum(Wk.algorithmic_bytes(it, es) for it in items)
+        b = F.Batch(items)
+        st = b.stats()
+        b.autotune(3)
+        for _ in range(3):
+            b.graph_launch()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(10):
+            b.graph_launch()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / 10
+        out[dname] = {"ms": ms, "gdofs": st["dofs"] / (ms * 1e-3) / 1e9,
+                      "achieved_GBs": nbytes / (ms * 1e-3) / 1e9,
+                      "frac_of_hbm_peak": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                      "groups": st["groups"], "cells": st["cells"]}
+        b.close()
+        del items
+        torch.cuda.empty_cache()
+    out["workload"] = ("200 meshes, cells lognormal(ln 1e4, 1) clipped to "
+                       "[1e3, 1e5], P uniform in 2..8, Q=P+1, half mass / half "
+                       "stiffness, continuous, hipGraph replay, autotuned")
+    return out
+
+
+def bench_serving(torch, F):
+    from femtc_amd import serving as Sv
+    items = Sv.make_pool(torch)
+    st, _ = Sv.run_stream(torch, items, rate=500.0, duration=2.0, slots=4)
+    st["workload"] = ("Poisson 500 req/s for 2 s, half applies / half (K+M)u=f "
+                      "solves (FP32 inner + FP64 refinement to 1e-10), 16 "
+                      "resident meshes n^3, n in 4..12, P in 2..6, 4 slots")
+    return st
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -142,6 +294,10 @@ def main():
     ap.add_argument("--n", type=int, default=48, help="cells per direction")
     ap.add_argument("--degree", type=int, default=6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--apply-only", action="store_true",
+                    help="skip the solve, config2, batch and serving parts")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
+                    help="only copy MiB three times (counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {a.gpus} is not supported "
@@ -154,54 +310,31 @@ def main():
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if a.copy_calibration:
+        m = a.copy_calibration * (1 << 20) // 8
+        src = torch.ones(m, device=dev, dtype=torch.float64)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": 8 * m}), flush=True)
+        return
+
     n, P = a.n, a.degree
-    Q = P + 1
-    N = P + 1
-    C = n ** 3
-    ng = (n * P + 1) ** 3
+    pr = structured(torch, F, n, P)
+    C, ng, N, Q = pr["C"], pr["ng"], pr["N"], pr["Q"]
     print(f"[bench] mesh {n}^3 P={P} Q={Q} cells={C} dofs={ng}", file=sys.stderr,
           flush=True)
-    stream = torch.cuda.current_stream()
-    tab = torch.from_numpy(F.basis_tables(P, Q)).to(dev)
-    J = _uniform(3, 9 * C, -1, 1) * 0.1
-    J = (J.reshape(C, 3, 3) + np.eye(3)).ravel()
-    dJ = torch.from_numpy(J).to(dev)
-    G = torch.empty(C * Q ** 3 * 6, device=dev, dtype=torch.float64)
-    F.geometry_affine(dJ, Q, G=G)
-    dm = torch.empty(C * N ** 3, device=dev, dtype=torch.int32)
-    F.dofmap_structured(n, n, n, P, dm)
     x = torch.from_numpy(_uniform(4, ng, -1, 1)).to(dev)
     y = torch.empty(ng, device=dev, dtype=torch.float64)
-    del dJ
     torch.cuda.synchronize()
-
-    def apply():
-        # y += K x into the freshly cleared y
-        F.global_apply(F.OP_STIFFNESS, tab, x, y, dm, C, P, Q, G=G,
-                       accumulate=True)
-
-    for _ in range(a.warmup):
-        F.fill_zero(y)
-        apply()
-    torch.cuda.synchronize()
-    ev = [(torch.cuda.Event(enable_timing=True),
-           torch.cuda.Event(enable_timing=True)) for _ in range(a.steps)]
-    t0 = time.perf_counter()
-    for i in range(a.steps):
-        F.fill_zero(y)
-        ev[i][0].record(stream)
-        apply()
-        ev[i][1].record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    kern_ms = float(np.mean([s.elapsed_time(e) for s, e in ev]))
-    ms_per_step = wall * 1e3 / a.steps
+    ms_per_step, kern_ms = time_apply(torch, F, pr, a.steps, a.warmup, x, y)
     value = ng / (ms_per_step * 1e-3) / 1e9
-
-    alg_bytes = 8 * ng * 2 + 4 * C * N ** 3 + 8 * 6 * C * Q ** 3
+    alg_bytes = alg_bytes_stiffness(pr)
     achieved = alg_bytes / (kern_ms * 1e-3) / 1e9
     # a sanity check of the result: y^T 1 = x^T K 1 = 0 up to rounding
     s = float(y.sum().item())
+    traffic = measured_traffic() if (n, P) == (48, 6) else None
     out = {
         "metric": METRIC,
         "value": value,
@@ -227,7 +360,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
             "kernel": f"apply_cells<double,{N},{Q},OP_STIFF,gather>",
             "kernel_ms": kern_ms,
             "algorithmic_bytes": alg_bytes,
@@ -235,6 +368,15 @@ def main():
         "cell_local_gdofs": C * N ** 3 / (ms_per_step * 1e-3) / 1e9,
         "check_sum_y": s,
     }
+    del x, y, pr
+    torch.cuda.empty_cache()
+    if not a.apply_only:
+        for key, fn in (("solve", bench_solve), ("config2", bench_config2),
+                        ("batch", bench_batch), ("serving", bench_serving)):
+            print(f"[bench] {key}", file=sys.stderr, flush=True)
+            out[key] = fn(torch, F)
+            torch.cuda.empty_cache()
+        out["p99_solve_ms"] = out["serving"].get("solve_p99_ms")
     if not a.no_cpu_baseline:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(P, C, ng)

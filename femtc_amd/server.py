@@ -120,9 +120,13 @@ def latency_stats(results) -> dict:
     lat = np.array([r["t_end"] - r["t_arrival"] for r in results]) * 1e3
     span = max(r["t_end"] for r in results) - min(r["t_arrival"]
                                                    for r in results)
-    solves = sum(r["kind"] == _lib.REQUEST_SOLVE for r in results)
-    return {"n": len(results), "p50_ms": float(np.percentile(lat, 50)),
-            "p99_ms": float(np.percentile(lat, 99)),
-            "mean_ms": float(lat.mean()),
-            "requests_per_s": len(results) / span if span > 0 else None,
-            "solves_per_s": solves / span if span > 0 else None}
+    is_solve = np.array([r["kind"] == _lib.REQUEST_SOLVE for r in results])
+    out = {"n": len(results), "p50_ms": float(np.percentile(lat, 50)),
+           "p99_ms": float(np.percentile(lat, 99)),
+           "mean_ms": float(lat.mean()),
+           "requests_per_s": len(results) / span if span > 0 else None,
+           "solves_per_s": int(is_solve.sum()) / span if span > 0 else None}
+    if is_solve.any():
+        out["solve_p50_ms"] = float(np.percentile(lat[is_solve], 50))
+        out["solve_p99_ms"] = float(np.percentile(lat[is_solve], 99))
+    return out
