@@ -7,10 +7,11 @@ library; the first call does, and fails loudly if it is not built.
 from ._lib import (BF16, F32, F64, MODE_CORRECT, MODE_REFERENCE, OP_KPM,
                    OP_MASS, OP_STIFFNESS, FemtcError, lib)
 from .linalg import Order, string2order
-from .mesh import (basis_tables, dofmap_structured, gauss_rule, geometry_affine,
-                   gll_points)
-from .operators import (fill_zero, global_apply, kpm_apply, mass_apply,
-                        mass_operator, stiffness_apply, stiffness_operator)
+from .mesh import (basis_tables, basis_tables_weights, dofmap_structured,
+                   gauss_rule, geometry_affine, geometry_cell, gll_points)
+from .operators import (fill_zero, global_apply, global_apply_affine,
+                        kpm_apply, mass_apply, mass_operator, stiffness_apply,
+                        stiffness_operator)
 from .batch import Batch
 from .plan import Plan
 from .server import ResidentProblem, Server, latency_stats

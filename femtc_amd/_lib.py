@@ -66,6 +66,10 @@ SIGNATURES = {
                                         _i32, _i32, _i32, _i32, _i32]),
     "femtc_dofmap_structured": (_i32, [_i32, _i32, _i32, _i32, _vp, _vp]),
     "femtc_geometry_affine": (_i32, [_i64, _i32, _vp, _vp, _vp, _i32, _vp]),
+    "femtc_geometry_cell": (_i32, [_i64, _vp, _vp, _i32, _vp]),
+    "femtc_basis_tables_weights": (_i32, [_i32, _i32, _dp]),
+    "femtc_global_apply_affine": (_i32, [_i32, _i32, _i32, _i32, _vp, _vp, _vp,
+                                         _i64, _vp, _vp, _i64, _i32, _vp]),
     "femtc_plan_create": (_i32, [C.POINTER(_vp), _i32, _i32, _i32, _i32, _vp,
                                  _i64, _vp]),
     "femtc_plan_destroy": (_i32, [_vp]),
@@ -105,7 +109,7 @@ class Problem(C.Structure):
     """femtc_problem (include/femtc.h)."""
     _fields_ = [("op", _i32), ("degree", _i32), ("nq", _i32),
                 ("num_cells", _i64), ("ndofs", _i64), ("dofmap", _vp),
-                ("tables", _vp), ("G", _vp), ("detJw", _vp)]
+                ("tables", _vp), ("G", _vp), ("detJw", _vp), ("J", _vp)]
 
 
 class BatchItem(C.Structure):

@@ -3,6 +3,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "common.hpp"
 
@@ -66,7 +67,8 @@ int run(const ApplyReq& r)
 {
   // The kernels read a cell's geometry block as pairs of elements.
   const uintptr_t galign = 2 * (r.dtype == FEMTC_F64 ? 8 : 4);
-  if (r.op != OP_MASS && r.G && (uintptr_t)r.G % galign != 0)
+  if ((r.op == OP_STIFF || r.op == OP_KPM || r.op == OP_STIFF_LIT) && r.G &&
+      (uintptr_t)r.G % galign != 0)
     return fail(FEMTC_ERR_INVALID_ARGUMENT,
                 "G must be aligned to two elements (16 bytes for FP64, 8 for "
                 "FP32)");
@@ -309,6 +311,58 @@ int femtc_global_apply_add(int op, int dtype, int degree, int nq,
 {
   return global_apply_impl(op, dtype, degree, nq, tables, x, y, ndofs, dofmap,
                            G, detJw, num_cells, stream, true);
+}
+
+int femtc_global_apply_affine(int op, int dtype, int degree, int nq,
+                              const void* tables, const void* x, void* y,
+                              int64_t ndofs, const int32_t* dofmap,
+                              const void* cellgeo, int64_t num_cells,
+                              int accumulate, void* stream)
+{
+  int st = check_shape(degree, nq, FEMTC_MODE_CORRECT);
+  if (st || (st = check_dtype(dtype)))
+    return st;
+  if (op != FEMTC_OP_MASS && op != FEMTC_OP_STIFFNESS && op != FEMTC_OP_KPM)
+    return fail(FEMTC_ERR_INVALID_ARGUMENT, "op must be a femtc_op");
+  if (num_cells < 0 || ndofs < 0)
+    return fail(FEMTC_ERR_INVALID_ARGUMENT, "negative size");
+  if (ndofs > 0 && (!x || !y))
+    return fail(FEMTC_ERR_INVALID_ARGUMENT, "null vector");
+  if (num_cells > 0 && (!tables || !dofmap || !cellgeo))
+    return fail(FEMTC_ERR_INVALID_ARGUMENT,
+                "null tables, dofmap or cell geometry");
+  hipStream_t s = (hipStream_t)stream;
+  if (!accumulate && (st = fill_zero(y, ndofs, dtype, s)))
+    return fail(st, "fill kernel launch failed");
+  if (num_cells == 0)
+    return FEMTC_OK;
+  ApplyReq r;
+  r.op = op == FEMTC_OP_MASS
+             ? OP_MASS_AFF
+             : (op == FEMTC_OP_STIFFNESS ? OP_STIFF_AFF : OP_KPM_AFF);
+  r.dtype = dtype;
+  r.N = degree + 1;
+  r.Q = nq;
+  r.gather = 1;
+  r.tab = tables;
+  r.x = x;
+  r.y = y;
+  r.G = cellgeo;
+  r.dofmap = dofmap;
+  r.num_cells = num_cells;
+  r.stream = s;
+  return run(r);
+}
+
+int femtc_basis_tables_weights(int degree, int nq, double* tables)
+{
+  if (!tables)
+    return fail(FEMTC_ERR_INVALID_ARGUMENT, "tables is NULL");
+  int st = femtc_basis_tables(degree, nq, tables);
+  if (st != FEMTC_OK)
+    return st;
+  std::vector<double> xq(nq);
+  return femtc_gauss_rule(nq, xq.data(), tables + 2 * (degree + 1) * nq);
 }
 
 int femtc_fill_zero(void* y, int64_t n, int dtype, void* stream)

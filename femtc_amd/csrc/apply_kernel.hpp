@@ -160,14 +160,17 @@ __device__ inline void load_geometry_half(const T* __restrict__ G,
 template <typename T, int N, int Q, int OP>
 constexpr int work_lds_elems()
 {
-  constexpr bool GRAD = OP != OP_MASS;
+  constexpr bool AFF = OP >= OP_MASS_AFF;
+  constexpr int OPB = AFF ? OP - OP_MASS_AFF : OP;
+  constexpr bool GRAD = OPB != OP_MASS;
+  constexpr bool GEO = GRAD && !AFF;
   constexpr int CPB = cells_per_block<Q>();
   constexpr LdsLayout LY = lds_layout<(int)sizeof(T), N, Q>();
   constexpr int XCELLS = CPB > 1 ? LY.XCELL : (GRAD ? 2 : 1) * Q * LY.XQ;
   constexpr int YCELLS = CPB > 1 ? LY.YCELL : (GRAD ? 3 : 1) * N * LY.YA;
   constexpr int WCELL = XCELLS > YCELLS ? XCELLS : YCELLS;
   constexpr int WORK = CPB * WCELL;
-  constexpr int GELEM = GRAD ? 6 * geometry_split<Q>() * Q * Q * CPB : 0;
+  constexpr int GELEM = GEO ? 6 * geometry_split<Q>() * Q * Q * CPB : 0;
   return WORK > GELEM ? WORK : GELEM;
 }
 
@@ -225,10 +228,14 @@ apply_body(const T* __restrict__ tab, const T* __restrict__ x,
   constexpr int CPB = cells_per_block<Q>();
   constexpr int BT = BLK ? 64 : block_threads<Q>(); // threads per cell group
   constexpr int NL = Q * Q;
-  constexpr bool LIT = (OP == OP_STIFF_LIT);
-  constexpr bool MASSTERM = (OP == OP_MASS || OP == OP_KPM);
-  constexpr bool GRAD = (OP != OP_MASS);
-  constexpr bool FULL = (OP == OP_STIFF || OP == OP_KPM); // all 3 gradients
+  constexpr bool AFF = OP >= OP_MASS_AFF; // geometry from 7 values per cell
+  constexpr int OPB = AFF ? OP - OP_MASS_AFF : OP;
+  constexpr bool LIT = (OPB == OP_STIFF_LIT);
+  constexpr bool MASSTERM = (OPB == OP_MASS || OPB == OP_KPM);
+  constexpr bool GRAD = (OPB != OP_MASS);
+  constexpr bool FULL = (OPB == OP_STIFF || OPB == OP_KPM); // 3 gradients
+  constexpr bool GEO = GRAD && !AFF; // geometry streamed per point
+  static_assert(!AFF || (!LIT && !REF && !BLK), "affine geometry variants");
   constexpr bool UQ = MASSTERM || LIT;
   constexpr int NX = GRAD ? 2 : 1;
   constexpr int NY = GRAD ? 3 : 1;
@@ -247,8 +254,8 @@ apply_body(const T* __restrict__ tab, const T* __restrict__ x,
   // geometry of the workgroup in pieces of two elements, in two halves of
   // qa-planes: [0, QA1) and [QA1, Q); GP0/GP1 pieces, GPT0/GPT1 per thread
   constexpr int QA1 = geometry_split<Q>();
-  constexpr int GP0 = GRAD ? 3 * QA1 * Q2 * CPB : 0;
-  constexpr int GP1 = GRAD ? 3 * (Q - QA1) * Q2 * CPB : 0;
+  constexpr int GP0 = GEO ? 3 * QA1 * Q2 * CPB : 0;
+  constexpr int GP1 = GEO ? 3 * (Q - QA1) * Q2 * CPB : 0;
   constexpr int GPT0 = (GP0 + BT - 1) / BT;
   constexpr int GPT1 = (GP1 + BT - 1) / BT;
   // LDS: X and Y share one area per cell (WCELL); a geometry half aliases
@@ -362,15 +369,30 @@ apply_body(const T* __restrict__ tab, const T* __restrict__ x,
           u[b] = x[cell * N3 + uoff + b * N];
       }
     }
-    Piece<T> gp0[GRAD ? GPT0 : 1], gp1[GRAD ? GPT1 : 1];
-    if constexpr (GRAD)
+    Piece<T> gp0[GEO ? GPT0 : 1], gp1[GEO ? GPT1 : 1];
+    if constexpr (GEO)
     {
       load_geometry_half<T, Q, BT, 0, QA1>(G, cell0, ncells_blk, tid, gp0);
       load_geometry_half<T, Q, BT, QA1, Q - QA1>(G, cell0, ncells_blk, tid,
                                                   gp1);
     }
+    // affine cells: S = detJ J^-1 J^-T (packed as G) and detJ of the cell,
+    // and w(qb) w(qc) of this thread's quadrature line
+    T cg[AFF ? CELLGEO : 1];
+    T wbc = T(1);
+    if constexpr (AFF)
+    {
+      if (live)
+      {
+#pragma unroll
+        for (int k = 0; k < CELLGEO; ++k)
+          cg[k] = G[cell * CELLGEO + k];
+        const T* wq = tab + 2 * N * Q;
+        wbc = wq[ln / Q] * wq[ln - (ln / Q) * Q];
+      }
+    }
     T dw[MASSTERM ? Q : 1];
-    if constexpr (MASSTERM)
+    if constexpr (MASSTERM && !AFF)
     {
       if (live)
       {
@@ -491,7 +513,7 @@ apply_body(const T* __restrict__ tab, const T* __restrict__ x,
     }
     stage_sync<BLK>(); // X and Y are idle from here until B3 writes Y
     // first geometry half into LDS (its registers are freed before F3)
-    if constexpr (GRAD)
+    if constexpr (GEO)
     {
 #pragma unroll
       for (int k = 0; k < GPT0; ++k)
@@ -535,7 +557,7 @@ apply_body(const T* __restrict__ tab, const T* __restrict__ x,
     T va[Q], vb[Q], vc[Q], vm[Q];
     auto pointwise = [&](int q, int qa0, int per_cell) {
       T g[6];
-      if constexpr (GRAD)
+      if constexpr (GEO)
       {
         const Piece<T>* Lg =
             Ls + lc * per_cell + ((q - qa0) * Q2 + ln) * 3;
@@ -546,6 +568,19 @@ apply_body(const T* __restrict__ tab, const T* __restrict__ x,
           g[2 * j] = pc.x;
           g[2 * j + 1] = pc.y;
         }
+      }
+      if constexpr (AFF)
+      {
+        // G(q) = w(qa) w(qb) w(qc) S, detJw(q) = w(qa) w(qb) w(qc) detJ
+        const T wq = tab[2 * N * Q + q] * wbc;
+        if constexpr (GRAD)
+        {
+#pragma unroll
+          for (int k = 0; k < 6; ++k)
+            g[k] = wq * cg[k];
+        }
+        if constexpr (MASSTERM)
+          dw[q] = wq * cg[6];
       }
       if constexpr (FULL)
       {
@@ -569,7 +604,7 @@ apply_body(const T* __restrict__ tab, const T* __restrict__ x,
       for (int q = 0; q < QA1; ++q)
         pointwise(q, 0, 3 * QA1 * Q2);
     }
-    if constexpr (GRAD)
+    if constexpr (GEO)
     {
       stage_sync<BLK>();
 #pragma unroll
@@ -614,7 +649,7 @@ apply_body(const T* __restrict__ tab, const T* __restrict__ x,
         rc_[a] = rc;
       }
     }
-    if constexpr (GRAD)
+    if constexpr (GEO)
       stage_sync<BLK>(); // the geometry in LDS is consumed; Y may be rewritten
     if (live)
     {
@@ -995,6 +1030,19 @@ int dispatch_op(const ApplyReq& r)
       return FEMTC_ERR_INVALID_ARGUMENT;
     return g ? launch_cells<T, N, Q, OP_KPM, false, true>(r)
              : launch_cells<T, N, Q, OP_KPM, false, false>(r);
+  case OP_MASS_AFF:
+  case OP_STIFF_AFF:
+  case OP_KPM_AFF:
+    if (r.reference_mode)
+      return FEMTC_ERR_INVALID_ARGUMENT;
+    if (r.op == OP_MASS_AFF)
+      return g ? launch_cells<T, N, Q, OP_MASS_AFF, false, true>(r)
+               : launch_cells<T, N, Q, OP_MASS_AFF, false, false>(r);
+    if (r.op == OP_STIFF_AFF)
+      return g ? launch_cells<T, N, Q, OP_STIFF_AFF, false, true>(r)
+               : launch_cells<T, N, Q, OP_STIFF_AFF, false, false>(r);
+    return g ? launch_cells<T, N, Q, OP_KPM_AFF, false, true>(r)
+             : launch_cells<T, N, Q, OP_KPM_AFF, false, false>(r);
   }
   return FEMTC_ERR_INVALID_ARGUMENT;
 }

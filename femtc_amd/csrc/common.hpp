@@ -18,7 +18,16 @@ enum Op : int
   OP_STIFF = 1,      // sum_ij B_i^T S_ij B_j
   OP_KPM = 2,        // stiffness + mass
   OP_STIFF_LIT = 3,  // the reference's literal stiffness (src/stiffness.hpp:34-99)
+  // the same operators on affine cells with the geometry computed in the
+  // kernel from 7 values per cell (detJ J^-1 J^-T packed as G, and detJ)
+  // times the tensor-product quadrature weights
+  OP_MASS_AFF = 4,
+  OP_STIFF_AFF = 5,
+  OP_KPM_AFF = 6,
 };
+
+// Values per cell of the affine geometry: G's packing, then detJ.
+constexpr int CELLGEO = 7;
 
 // One operator application request, filled by the C-ABI layer and consumed
 // by the per-degree launchers.

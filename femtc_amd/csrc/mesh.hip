@@ -39,6 +39,42 @@ struct Weights1D
   double w[16];
 };
 
+// J^-1 of a 3x3 row-major J by cofactors (FP64); returns det J.
+__device__ inline double inverse3(const double* j, double (&inv)[9])
+{
+  const double c00 = j[4] * j[8] - j[5] * j[7];
+  const double c01 = j[5] * j[6] - j[3] * j[8];
+  const double c02 = j[3] * j[7] - j[4] * j[6];
+  const double det = j[0] * c00 + j[1] * c01 + j[2] * c02;
+  const double id = 1.0 / det;
+  inv[0] = c00 * id;
+  inv[1] = (j[2] * j[7] - j[1] * j[8]) * id;
+  inv[2] = (j[1] * j[5] - j[2] * j[4]) * id;
+  inv[3] = c01 * id;
+  inv[4] = (j[0] * j[8] - j[2] * j[6]) * id;
+  inv[5] = (j[2] * j[3] - j[0] * j[5]) * id;
+  inv[6] = c02 * id;
+  inv[7] = (j[1] * j[6] - j[0] * j[7]) * id;
+  inv[8] = (j[0] * j[4] - j[1] * j[3]) * id;
+  return det;
+}
+
+// s J^-1 J^-T in the packed order (S22, S12, S02, S11, S01, S00).
+template <typename T>
+__device__ inline void store_packed(const double (&inv)[9], double s, T* g)
+{
+  auto K = [&](int r, int t) {
+    return inv[3 * r] * inv[3 * t] + inv[3 * r + 1] * inv[3 * t + 1] +
+           inv[3 * r + 2] * inv[3 * t + 2];
+  };
+  g[0] = (T)(s * K(2, 2));
+  g[1] = (T)(s * K(1, 2));
+  g[2] = (T)(s * K(0, 2));
+  g[3] = (T)(s * K(1, 1));
+  g[4] = (T)(s * K(0, 1));
+  g[5] = (T)(s * K(0, 0));
+}
+
 // One thread per (cell, point): invert the per-cell constant Jacobian
 // (cofactors, FP64) and scale J^-1 J^-T by |det J| w_q.
 template <typename T>
@@ -54,39 +90,30 @@ __global__ void __launch_bounds__(256)
     const long long cell = i / Q3;
     const int q = (int)(i - cell * Q3);
     const int qa = q / (Q * Q), qb = (q / Q) % Q, qc = q % Q;
-    const double* j = J + 9 * cell;
-    const double c00 = j[4] * j[8] - j[5] * j[7];
-    const double c01 = j[5] * j[6] - j[3] * j[8];
-    const double c02 = j[3] * j[7] - j[4] * j[6];
-    const double det = j[0] * c00 + j[1] * c01 + j[2] * c02;
-    const double id = 1.0 / det;
     double inv[9];
-    inv[0] = c00 * id;
-    inv[1] = (j[2] * j[7] - j[1] * j[8]) * id;
-    inv[2] = (j[1] * j[5] - j[2] * j[4]) * id;
-    inv[3] = c01 * id;
-    inv[4] = (j[0] * j[8] - j[2] * j[6]) * id;
-    inv[5] = (j[2] * j[3] - j[0] * j[5]) * id;
-    inv[6] = c02 * id;
-    inv[7] = (j[1] * j[6] - j[0] * j[7]) * id;
-    inv[8] = (j[0] * j[4] - j[1] * j[3]) * id;
+    const double det = inverse3(J + 9 * cell, inv);
     const double s = fabs(det) * w1.w[qa] * w1.w[qb] * w1.w[qc];
     if (G)
-    {
-      auto K = [&](int r, int t) {
-        return inv[3 * r] * inv[3 * t] + inv[3 * r + 1] * inv[3 * t + 1] +
-               inv[3 * r + 2] * inv[3 * t + 2];
-      };
-      T* g = G + i * 6;
-      g[0] = (T)(s * K(2, 2));
-      g[1] = (T)(s * K(1, 2));
-      g[2] = (T)(s * K(0, 2));
-      g[3] = (T)(s * K(1, 1));
-      g[4] = (T)(s * K(0, 1));
-      g[5] = (T)(s * K(0, 0));
-    }
+      store_packed(inv, s, G + i * 6);
     if (detJw)
       detJw[i] = (T)s;
+  }
+}
+
+// One thread per cell: |det J| J^-1 J^-T (packed) and |det J|, the 7
+// values the affine operator variants scale by the quadrature weights.
+template <typename T>
+__global__ void __launch_bounds__(256)
+    cellgeo_kernel(const double* __restrict__ J, T* __restrict__ out,
+                   long long num_cells)
+{
+  for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+       c < num_cells; c += (long long)gridDim.x * blockDim.x)
+  {
+    double inv[9];
+    const double det = fabs(inverse3(J + 9 * c, inv));
+    store_packed(inv, det, out + c * CELLGEO);
+    out[c * CELLGEO + 6] = (T)det;
   }
 }
 
@@ -183,6 +210,34 @@ int femtc_geometry_affine(int64_t num_cells, int nq, const double* J, void* G,
   else
   {
     femtc::set_last_error("geometry_affine: dtype must be F64 or F32");
+    return FEMTC_ERR_INVALID_ARGUMENT;
+  }
+  return femtc::hip_status(hipGetLastError());
+}
+
+int femtc_geometry_cell(int64_t num_cells, const double* J, void* cellgeo,
+                        int dtype, void* stream)
+{
+  if (num_cells < 0 || (num_cells > 0 && (!J || !cellgeo)))
+  {
+    femtc::set_last_error("geometry_cell: invalid arguments");
+    return FEMTC_ERR_INVALID_ARGUMENT;
+  }
+  if (num_cells == 0)
+    return FEMTC_OK;
+  if (dtype == FEMTC_F64)
+    hipLaunchKernelGGL(femtc::cellgeo_kernel<double>,
+                       dim3(femtc::grid_for(num_cells)), dim3(256), 0,
+                       (hipStream_t)stream, J, (double*)cellgeo,
+                       (long long)num_cells);
+  else if (dtype == FEMTC_F32)
+    hipLaunchKernelGGL(femtc::cellgeo_kernel<float>,
+                       dim3(femtc::grid_for(num_cells)), dim3(256), 0,
+                       (hipStream_t)stream, J, (float*)cellgeo,
+                       (long long)num_cells);
+  else
+  {
+    femtc::set_last_error("geometry_cell: dtype must be F64 or F32");
     return FEMTC_ERR_INVALID_ARGUMENT;
   }
   return femtc::hip_status(hipGetLastError());

@@ -31,13 +31,20 @@ size_t solver_bytes(const femtc_problem& p, int inner)
   size_t b = 4 * 8 * n;                           // diag, dinv, r, A u (FP64)
   const size_t vs = inner == FEMTC_F64 ? 8 : 4;
   b += 4 * vs * n;                                // rhs, e, p, A p
+  if (p.J)
+    b += 8 * CELLGEO * C;                         // per-cell geometry, FP64
   if (inner != FEMTC_F64)
   {
     b += 2 * 4 * n;                               // r, D^-1 (FP32)
-    if (p.G)
-      b += 4 * 6 * q3 * C;
-    if (p.detJw)
-      b += 4 * q3 * C;
+    if (p.J)
+      b += 4 * CELLGEO * C;                       // per-cell geometry, FP32
+    else
+    {
+      if (p.G)
+        b += 4 * 6 * q3 * C;
+      if (p.detJw)
+        b += 4 * q3 * C;
+    }
   }
   return b + 64 * 1024; // partial sums, scalars, tables
 }

@@ -17,6 +17,7 @@
 // the iteration (the operator included) returns at once when it is set.  The
 // host checks `done` once per batch.
 #include <algorithm>
+#include <vector>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -352,8 +353,8 @@ template <typename T>
 __global__ void __launch_bounds__(256)
     k_diag(int op, int N, int Q, const T* __restrict__ tab,
            const T* __restrict__ G, const T* __restrict__ detJw,
-           const int32_t* __restrict__ dofmap, int64_t num_cells,
-           double* __restrict__ diag)
+           const T* __restrict__ cellgeo, const int32_t* __restrict__ dofmap,
+           int64_t num_cells, double* __restrict__ diag)
 {
   extern __shared__ double sm[];
   double* ph = sm;
@@ -361,11 +362,15 @@ __global__ void __launch_bounds__(256)
   double* F = gr + N * Q;
   double* T1 = F + Q * Q * Q;     // [qa][qb][c]
   double* T2 = T1 + Q * Q * N;    // [qa][b][c]
+  double* wt = T2 + Q * N * N;    // Gauss weights (affine cells)
   const int64_t cell = blockIdx.x;
   const int tid = threadIdx.x, nt = blockDim.x;
   const int Q3 = Q * Q * Q, N3 = N * N * N, NN = N * N;
-  for (int i = tid; i < 2 * N * Q; i += nt)
-    sm[i] = (double)tab[i];
+  // with cellgeo, tables are [phi | dphi | w] and the geometry of point q is
+  // w(qa) w(qb) w(qc) times the cell's 7 values
+  const int ntab = 2 * N * Q + (cellgeo ? Q : 0);
+  for (int i = tid; i < ntab; i += nt)
+    sm[i < 2 * N * Q ? i : (int)(wt - sm) + i - 2 * N * Q] = (double)tab[i];
   // term -> (G slot, factor, X, Y, Z); slot 6 = detJw
   const int slot[7] = {5, 3, 0, 4, 2, 1, 6};
   const double fac[7] = {1, 1, 1, 2, 2, 2, 1};
@@ -381,9 +386,16 @@ __global__ void __launch_bounds__(256)
     if (t == 6 && op == FEMTC_OP_STIFFNESS)
       continue;
     __syncthreads(); // previous term's readers are done
+    __syncthreads(); // tables and weights are in LDS
     for (int q = tid; q < Q3; q += nt)
-      F[q] = fac[t] * (slot[t] == 6 ? (double)detJw[cell * Q3 + q]
-                                    : (double)G[(cell * Q3 + q) * 6 + slot[t]]);
+    {
+      if (cellgeo)
+        F[q] = fac[t] * wt[q / (Q * Q)] * wt[(q / Q) % Q] * wt[q % Q] *
+               (double)cellgeo[cell * CELLGEO + slot[t]];
+      else
+        F[q] = fac[t] * (slot[t] == 6 ? (double)detJw[cell * Q3 + q]
+                                      : (double)G[(cell * Q3 + q) * 6 + slot[t]]);
+    }
     __syncthreads();
     for (int i = tid; i < Q * Q * N; i += nt)
     {
@@ -426,13 +438,14 @@ __global__ void __launch_bounds__(256)
 
 size_t diag_lds_bytes(int N, int Q)
 {
-  return sizeof(double) * (size_t)(2 * N * Q + Q * Q * Q + Q * Q * N + Q * N * N);
+  return sizeof(double) *
+         (size_t)(2 * N * Q + Q * Q * Q + Q * Q * N + Q * N * N + Q);
 }
 
 int launch_diag(int op, int dtype, int N, int Q, const void* tab,
                 const void* G, const void* detJw, const int32_t* dofmap,
                 int64_t num_cells, double* diag, int64_t ndofs,
-                hipStream_t s)
+                hipStream_t s, const void* cellgeo = nullptr)
 {
   int st = fill_zero(diag, ndofs, FEMTC_F64, s);
   if (st != FEMTC_OK)
@@ -445,12 +458,13 @@ int launch_diag(int op, int dtype, int N, int Q, const void* tab,
   if (dtype == FEMTC_F64)
     hipLaunchKernelGGL(k_diag<double>, dim3((unsigned)num_cells), dim3(256),
                        lds, s, op, N, Q, (const double*)tab,
-                       (const double*)G, (const double*)detJw, dofmap,
-                       num_cells, diag);
+                       (const double*)G, (const double*)detJw,
+                       (const double*)cellgeo, dofmap, num_cells, diag);
   else
     hipLaunchKernelGGL(k_diag<float>, dim3((unsigned)num_cells), dim3(256),
                        lds, s, op, N, Q, (const float*)tab, (const float*)G,
-                       (const float*)detJw, dofmap, num_cells, diag);
+                       (const float*)detJw, (const float*)cellgeo, dofmap,
+                       num_cells, diag);
   return hip_status(hipGetLastError());
 }
 
@@ -494,8 +508,15 @@ struct femtc_solver
   void* Ap = nullptr;
   void* dinv = nullptr;
   void* tab = nullptr;   // inner-precision tables and geometry
-  void* G = nullptr;
+  void* G = nullptr;     // per point, or per cell (inner_affine)
   void* detJw = nullptr;
+  // affine cells (problem.J given): the inner operator computes its
+  // geometry from 7 values per cell; so does the FP64 operator when the
+  // problem has no per-point geometry (outer_affine)
+  bool inner_affine = false;
+  bool outer_affine = false;
+  double* tabw64 = nullptr; // [phi | dphi | w], FP64
+  double* cg64 = nullptr;   // [C][7], FP64
   int check_every = 16;
 };
 
@@ -554,8 +575,15 @@ void destroy(femtc_solver* S)
     dev_free(S->G);
     dev_free(S->detJw);
   }
+  dev_free(S->tabw64);
+  dev_free(S->cg64);
   delete S;
 }
+
+// The operator request of the inner solve (precision V) or of the FP64
+// outer residual.
+ApplyReq inner_req(const femtc_solver* S, int dtype, const void* x, void* y);
+ApplyReq outer_req(const femtc_solver* S, const void* x, void* y);
 
 ApplyReq operator_req(const femtc_solver* S, int dtype, const void* tab,
                       const void* G, const void* detJw, const void* x,
@@ -576,6 +604,29 @@ ApplyReq operator_req(const femtc_solver* S, int dtype, const void* tab,
   a.num_cells = S->prob.num_cells;
   a.stream = S->stream;
   return a;
+}
+
+ApplyReq inner_req(const femtc_solver* S, int dtype, const void* x, void* y)
+{
+  ApplyReq a = operator_req(S, dtype, S->tab, S->G, S->detJw, x, y);
+  if (S->inner_affine)
+  {
+    a.op += OP_MASS_AFF; // the same operator with per-cell geometry
+    a.detJw = nullptr;
+  }
+  return a;
+}
+
+ApplyReq outer_req(const femtc_solver* S, const void* x, void* y)
+{
+  if (S->outer_affine)
+  {
+    ApplyReq a = operator_req(S, FEMTC_F64, S->tabw64, S->cg64, nullptr, x, y);
+    a.op += OP_MASS_AFF;
+    return a;
+  }
+  return operator_req(S, FEMTC_F64, S->prob.tables, S->prob.G, S->prob.detJw,
+                      x, y);
 }
 
 int sync_read(femtc_solver* S, void* host, const void* dev, size_t bytes)
@@ -614,7 +665,7 @@ int pcg_run(femtc_solver* S, double rtol, int maxit, int* iters, int* status)
   hipLaunchKernelGGL(k_reduce_start, dim3(1), dim3(RB), 0, s,
                      (const double*)S->part, S->st, rtol, maxit);
   LAUNCH_CHECK();
-  ApplyReq a = operator_req(S, dtype_of<V>(), S->tab, S->G, S->detJw, p, Ap);
+  ApplyReq a = inner_req(S, dtype_of<V>(), p, Ap);
   a.skip = &S->st->done;
   for (;;)
   {
@@ -655,8 +706,7 @@ int true_residual(femtc_solver* S, const double* f, const double* u,
   int st = fill_zero(S->Au64, S->n, FEMTC_F64, s);
   if (st != FEMTC_OK)
     return st;
-  ApplyReq a = operator_req(S, FEMTC_F64, S->prob.tables, S->prob.G,
-                            S->prob.detJw, u, S->Au64);
+  ApplyReq a = outer_req(S, u, S->Au64);
   st = run(a);
   if (st != FEMTC_OK)
     return st;
@@ -699,10 +749,12 @@ int check_problem(const femtc_problem* P)
     return fail(FEMTC_ERR_INVALID_ARGUMENT, "negative size");
   if (!P->tables || (P->num_cells > 0 && !P->dofmap))
     return fail(FEMTC_ERR_INVALID_ARGUMENT, "tables and dofmap are required");
-  if (P->op != FEMTC_OP_MASS && !P->G)
-    return fail(FEMTC_ERR_INVALID_ARGUMENT, "G is required for this op");
-  if (P->op != FEMTC_OP_STIFFNESS && !P->detJw)
-    return fail(FEMTC_ERR_INVALID_ARGUMENT, "detJw is required for this op");
+  if (P->op != FEMTC_OP_MASS && !P->G && !P->J)
+    return fail(FEMTC_ERR_INVALID_ARGUMENT,
+                "G (or the cell Jacobians J) is required for this op");
+  if (P->op != FEMTC_OP_STIFFNESS && !P->detJw && !P->J)
+    return fail(FEMTC_ERR_INVALID_ARGUMENT,
+                "detJw (or the cell Jacobians J) is required for this op");
   return FEMTC_OK;
 }
 
@@ -782,9 +834,38 @@ int femtc_solver_create(femtc_solver** out, const femtc_problem* prob,
       st = fail(FEMTC_ERR_OUT_OF_MEMORY, "pinned host allocation failed");
       break;
     }
+    // affine cells: [phi | dphi | w] and the 7 per-cell values in FP64
+    const bool aff = prob->J != nullptr;
+    S->inner_affine = aff;
+    S->outer_affine =
+        aff && ((prob->op != FEMTC_OP_MASS && !prob->G) ||
+                (prob->op != FEMTC_OP_STIFFNESS && !prob->detJw));
+    const int64_t ntab = 2 * (int64_t)S->N * S->Q;
+    if (aff)
+    {
+      if ((st = dev_alloc((void**)&S->tabw64, 8 * (ntab + S->Q))) ||
+          (st = dev_alloc((void**)&S->cg64, 8 * CELLGEO * C)))
+        break;
+      std::vector<double> xq(S->Q), wq(S->Q);
+      femtc_gauss_rule(S->Q, xq.data(), wq.data());
+      if (hipMemcpyAsync(S->tabw64, prob->tables, 8 * ntab,
+                         hipMemcpyDeviceToDevice, s) != hipSuccess ||
+          hipMemcpy(S->tabw64 + ntab, wq.data(), 8 * S->Q,
+                    hipMemcpyHostToDevice) != hipSuccess)
+      {
+        st = fail(FEMTC_ERR_HIP, "solver: table copy failed");
+        break;
+      }
+      if ((st = femtc_geometry_cell(C, prob->J, S->cg64, FEMTC_F64, s)))
+        break;
+    }
     // Jacobi preconditioner from the FP64 operator
-    st = launch_diag(prob->op, FEMTC_F64, S->N, S->Q, prob->tables, prob->G,
-                     prob->detJw, prob->dofmap, C, S->diag, n, s);
+    if (S->outer_affine)
+      st = launch_diag(prob->op, FEMTC_F64, S->N, S->Q, S->tabw64, nullptr,
+                       nullptr, prob->dofmap, C, S->diag, n, s, S->cg64);
+    else
+      st = launch_diag(prob->op, FEMTC_F64, S->N, S->Q, prob->tables,
+                       prob->G, prob->detJw, prob->dofmap, C, S->diag, n, s);
     if (st != FEMTC_OK)
       break;
     if ((st = fill_zero(S->dbad, 1, FEMTC_F32, s)) != FEMTC_OK)
@@ -795,9 +876,25 @@ int femtc_solver_create(femtc_solver** out, const femtc_problem* prob,
     {
       S->r = S->r64;
       S->dinv = S->dinv64;
-      S->tab = (void*)prob->tables;
-      S->G = (void*)prob->G;
-      S->detJw = (void*)prob->detJw;
+      S->tab = aff ? (void*)S->tabw64 : (void*)prob->tables;
+      S->G = aff ? (void*)S->cg64 : (void*)prob->G;
+      S->detJw = aff ? nullptr : (void*)prob->detJw;
+    }
+    else if (aff)
+    {
+      // FP32 inner operator on affine cells: 7 values per cell
+      if ((st = dev_alloc(&S->r, 4 * n)) || (st = dev_alloc(&S->dinv, 4 * n)) ||
+          (st = dev_alloc(&S->tab, 4 * (ntab + S->Q))) ||
+          (st = dev_alloc(&S->G, 4 * CELLGEO * C)))
+        break;
+      hipLaunchKernelGGL((k_convert<double, float>), dim3(NBLK), dim3(RB), 0,
+                         s, (const double*)S->dinv64, (float*)S->dinv, n);
+      hipLaunchKernelGGL((k_convert<double, float>), dim3(NBLK), dim3(RB), 0,
+                         s, (const double*)S->tabw64, (float*)S->tab,
+                         ntab + S->Q);
+      hipLaunchKernelGGL((k_convert<double, float>), dim3(NBLK), dim3(RB), 0,
+                         s, (const double*)S->cg64, (float*)S->G,
+                         (int64_t)CELLGEO * C);
     }
     else
     {

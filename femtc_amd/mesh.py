@@ -46,3 +46,20 @@ def geometry_affine(J, nq: int, G=None, detJw=None, stream=None) -> None:
     _lib.check(_lib.lib().femtc_geometry_affine(
         J.numel() // 9, nq, _ptr(J), _ptr(G), _ptr(detJw), _dtype_code(ref),
         _stream(stream)), "geometry_affine")
+
+
+def basis_tables_weights(degree: int, nq: int) -> np.ndarray:
+    """[phi | dphi | w]: the tables followed by the nq Gauss weights (the
+    table layout of the affine-geometry operators)."""
+    t = np.zeros(2 * (degree + 1) * nq + nq)
+    _lib.check(_lib.lib().femtc_basis_tables_weights(
+        degree, nq, t.ctypes.data_as(_lib._dp)), "basis_tables_weights")
+    return t
+
+
+def geometry_cell(J, cellgeo, stream=None) -> None:
+    """Per-cell Jacobians J (FP64 device tensor [C*9]) -> cellgeo [C*7]:
+    |det J| J^-1 J^-T packed as G, then |det J| (dtype of cellgeo)."""
+    _lib.check(_lib.lib().femtc_geometry_cell(
+        J.numel() // 9, _ptr(J), _ptr(cellgeo), _dtype_code(cellgeo),
+        _stream(stream)), "geometry_cell")

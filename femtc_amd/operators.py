@@ -146,3 +146,14 @@ def global_apply(op: int, tables, x, y, dofmap, num_cells: int, degree: int,
         op, _dtype_code(x), degree, nq, _ptr(tables), _ptr(x), _ptr(y),
         y.numel(), _ptr(dofmap), _ptr(G), _ptr(detJw), num_cells,
         _stream(stream)), "global_apply")
+
+
+def global_apply_affine(op: int, tables_w, x, y, dofmap, num_cells: int,
+                        degree: int, nq: int, cellgeo, stream=None,
+                        accumulate: bool = False) -> None:
+    """y = A x (or y += A x) on affine cells with the geometry computed in
+    the kernel from cellgeo [C][7]; tables_w = [phi | dphi | w]."""
+    _lib.check(_lib.lib().femtc_global_apply_affine(
+        op, _dtype_code(x), degree, nq, _ptr(tables_w), _ptr(x), _ptr(y),
+        y.numel(), _ptr(dofmap), _ptr(cellgeo), num_cells, int(accumulate),
+        _stream(stream)), "global_apply_affine")

@@ -19,19 +19,20 @@ class ResidentProblem:
     detJw (FP64 torch tensors), num_cells, ndofs."""
 
     def __init__(self, op, degree, nq, tables, dofmap, num_cells, ndofs,
-                 G=None, detJw=None):
-        self._keep = (tables, dofmap, G, detJw)
+                 G=None, detJw=None, J=None):
+        self._keep = (tables, dofmap, G, detJw, J)
         self.ndofs = ndofs
         self.struct = _lib.Problem(op, degree, nq, num_cells, ndofs,
                                    _ptr(dofmap), _ptr(tables), _ptr(G),
-                                   _ptr(detJw))
+                                   _ptr(detJw), _ptr(J))
 
     @classmethod
-    def from_item(cls, it):
-        """From a femtc_amd.workloads.structured_problem dict."""
+    def from_item(cls, it, affine: bool = False):
+        """From a femtc_amd.workloads.structured_problem dict; affine=True
+        also hands the cell Jacobians to the solvers."""
         return cls(it["op"], it["degree"], it["nq"], it["tables"],
                    it["dofmap"], it["num_cells"], it["ndofs"], it.get("G"),
-                   it.get("detJw"))
+                   it.get("detJw"), it.get("J") if affine else None)
 
 
 class Server:

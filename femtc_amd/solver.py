@@ -32,13 +32,15 @@ class PCGSolver:
 
     def __init__(self, op: int, tables, dofmap, num_cells: int, degree: int,
                  nq: int, ndofs: int, G=None, detJw=None, inner: int = _lib.F64,
-                 stream=None):
-        self._keep = (tables, dofmap, G, detJw)
+                 stream=None, J=None):
+        """J: optional per-cell Jacobians [C*9] (FP64) of affine cells; the
+        inner operator then computes its geometry in the kernel."""
+        self._keep = (tables, dofmap, G, detJw, J)
         self.ndofs = ndofs
         self.inner = inner
         self.problem = _lib.Problem(op, degree, nq, num_cells, ndofs,
                                     _ptr(dofmap), _ptr(tables), _ptr(G),
-                                    _ptr(detJw))
+                                    _ptr(detJw), _ptr(J))
         h = C.c_void_p()
         _lib.check(_lib.lib().femtc_solver_create(
             C.byref(h), C.byref(self.problem), inner, _stream(stream)),

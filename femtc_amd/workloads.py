@@ -56,9 +56,8 @@ def structured_problem(torch, dims, degree: int, nq: int, op: int, dtype,
     if op != _lib.OP_STIFFNESS:
         D = torch.empty(C * nq ** 3, device=device, dtype=dtype)
     geometry_affine(J, nq, G=G, detJw=D)
-    del J
     item = {"op": op, "degree": degree, "nq": nq, "tables": tab, "G": G,
-            "detJw": D, "num_cells": C, "dims": (nx, ny, nz)}
+            "detJw": D, "J": J, "num_cells": C, "dims": (nx, ny, nz)}
     if continuous:
         ng = (nx * degree + 1) * (ny * degree + 1) * (nz * degree + 1)
         dm = torch.empty(C * N ** 3, device=device, dtype=torch.int32)

@@ -162,6 +162,26 @@ int femtc_global_apply_add(int op, int dtype, int degree, int nq,
 int femtc_fill_zero(void* y, int64_t n, int dtype, void* stream);
 
 /* ---------------------------------------------------------------------- */
+/* Affine cells, geometry computed in the kernel (no counterpart in the    */
+/* reference, which streams G per point): per cell 7 values cellgeo =      */
+/* (|det J| J^-1 J^-T packed as G, |det J|), scaled in the kernel by the   */
+/* tensor-product Gauss weights.  Equals the stored-geometry operator for  */
+/* per-cell constant J while reading 7 values per cell instead of 7 nq^3.  */
+/* tables here hold [phi | dphi | w]: 2*N*nq + nq values.                  */
+/* ---------------------------------------------------------------------- */
+/* J [num_cells][9] (FP64) -> cellgeo [num_cells][7] in dtype. */
+int femtc_geometry_cell(int64_t num_cells, const double* J, void* cellgeo,
+                        int dtype, void* stream);
+/* [phi | dphi | w] (host, FP64), 2*N*nq + nq values. */
+int femtc_basis_tables_weights(int degree, int nq, double* tables);
+/* y = A x (accumulate = 0, y overwritten) or y += A x through the dofmap. */
+int femtc_global_apply_affine(int op, int dtype, int degree, int nq,
+                              const void* tables, const void* x, void* y,
+                              int64_t ndofs, const int32_t* dofmap,
+                              const void* cellgeo, int64_t num_cells,
+                              int accumulate, void* stream);
+
+/* ---------------------------------------------------------------------- */
 /* Reference-shaped synchronous entry points on HOST buffers: the C side   */
 /* of the operators::mass_operator<T> / stiffness_operator<T> shims        */
 /* (src/mass.hpp:196-199, src/stiffness.hpp:245-248).  Q = degree+1 as in  */
@@ -302,6 +322,12 @@ typedef struct femtc_problem
   const double* tables; /* [phi | dphi], 2*N*nq */
   const double* G;      /* [num_cells][nq^3][6], or NULL for the mass op */
   const double* detJw;  /* [num_cells][nq^3], or NULL for the stiffness op */
+  /* Optional per-cell Jacobians [num_cells][9] (FP64, row-major) of affine
+   * cells.  When given, the inner operator computes its geometry from 7
+   * values per cell (femtc_global_apply_affine) instead of streaming G and
+   * detJw; G/detJw may then be NULL, and the FP64 residual uses the
+   * per-cell geometry too. */
+  const double* J;
 } femtc_problem;
 
 typedef struct femtc_solve_info

@@ -111,7 +111,7 @@ def test_solver_create_rejects_bad_problems_without_touching_the_device():
     p = _lib.Problem(F.OP_STIFFNESS, 3, 4, 10, 100, 1, 1, None, None)
     assert L.femtc_solver_create(C_.byref(h), C_.byref(p), F.F64, None) == \
         _lib.ERR_INVALID_ARGUMENT
-    assert "G is required" in _lib.last_error()
+    assert "G (or the cell Jacobians J) is required" in _lib.last_error()
     p = _lib.Problem(F.OP_MASS, 9, 10, 10, 100, 1, 1, None, 1)
     assert L.femtc_solver_create(C_.byref(h), C_.byref(p), F.F64, None) == \
         _lib.ERR_DEGREE_NOT_SUPPORTED
@@ -186,3 +186,14 @@ def test_latency_stats():
     assert st["p99_ms"] > 985.0
     assert abs(st["solves_per_s"] - 50.0) < 1e-9
     assert F.latency_stats([]) == {"n": 0}
+
+
+def test_basis_tables_weights_host():
+    for P, Q in ((1, 2), (4, 5), (6, 9)):
+        t = F.basis_tables_weights(P, Q)
+        N = P + 1
+        np.testing.assert_allclose(t[:2 * N * Q], O.basis_tables(P, Q),
+                                   rtol=0, atol=1e-13)
+        _, w = O.gl_rule(Q)
+        np.testing.assert_allclose(t[2 * N * Q:], w, rtol=1e-13)
+        assert abs(t[2 * N * Q:].sum() - 1.0) < 1e-14
