@@ -221,7 +221,8 @@ __device__ __forceinline__ void
 apply_body(const T* __restrict__ tab, const T* __restrict__ x,
            T* __restrict__ y, const T* __restrict__ G,
            const T* __restrict__ detJw, const int32_t* __restrict__ dofmap,
-           long long num_cells, const BlockArgs& ba, long long blk)
+           long long num_cells, const BlockArgs& ba, long long blk,
+           double* __restrict__ epart = nullptr)
 {
   constexpr bool BLK = BW > 0;
   static_assert(!BLK || (GATHER && Q * Q <= 64), "blocked scatter layout");
@@ -555,6 +556,7 @@ apply_body(const T* __restrict__ tab, const T* __restrict__ x,
     // pointwise: v = S(q) d (+ detJw u), the geometry staged through LDS one
     // half of the qa-planes at a time
     T va[Q], vb[Q], vc[Q], vm[Q];
+    double eacc = 0.0; // this thread's share of x^T A x (live cells only)
     auto pointwise = [&](int q, int qa0, int per_cell) {
       T g[6];
       if constexpr (GEO)
@@ -597,6 +599,11 @@ apply_body(const T* __restrict__ tab, const T* __restrict__ x,
       }
       if constexpr (MASSTERM)
         vm[q] = dw[q] * uq[q];
+      // energy x_e^T A_e x_e = sum_q d.S d + detJw u^2 (the CG's p.Ap)
+      if constexpr (FULL)
+        eacc += (double)(da[q] * va[q] + db[q] * vb[q] + dc[q] * vc[q]);
+      if constexpr (MASSTERM)
+        eacc += (double)(uq[q] * vm[q]);
     };
     if (live)
     {
@@ -651,6 +658,18 @@ apply_body(const T* __restrict__ tab, const T* __restrict__ x,
     }
     if constexpr (GEO)
       stage_sync<BLK>(); // the geometry in LDS is consumed; Y may be rewritten
+    // energy partial of this wave (fixed order: lane tree, one slot per wave)
+    if constexpr (!BLK && !LIT)
+    {
+      if (epart)
+      {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1)
+          eacc += __shfl_xor(eacc, o, 64);
+        if ((threadIdx.x & 63) == 0)
+          epart[blk * (BT / 64) + (threadIdx.x >> 6)] = eacc;
+      }
+    }
     if (live)
     {
 #pragma unroll
@@ -807,12 +826,12 @@ __global__ void __launch_bounds__(BW ? 64 * BW : block_threads<Q>(),
                 T* __restrict__ y, const T* __restrict__ G,
                 const T* __restrict__ detJw, const int32_t* __restrict__ dofmap,
                 long long num_cells, const int* __restrict__ skip,
-                BlockArgs ba)
+                BlockArgs ba, double* __restrict__ epart)
 {
   if (skip && *skip)
     return;
   apply_body<T, N, Q, OP, REF, GATHER, BW>(tab, x, y, G, detJw, dofmap,
-                                           num_cells, ba, blockIdx.x);
+                                           num_cells, ba, blockIdx.x, epart);
 }
 
 // Batched launch: many independent problems of one (T, N, Q, OP, GATHER)
@@ -928,7 +947,8 @@ int launch_cells(const ApplyReq& r)
                      dim3((unsigned)nblk), dim3(block_threads<Q>()), 0,
                      r.stream, (const T*)r.tab, (const T*)r.x, (T*)r.y,
                      (const T*)r.G, (const T*)r.detJw, r.dofmap,
-                     (long long)r.num_cells, r.skip, BlockArgs{});
+                     (long long)r.num_cells, r.skip, BlockArgs{},
+                     r.energy);
   return hip_status(hipGetLastError());
 }
 
@@ -982,7 +1002,7 @@ int launch_blocked(const ApplyReq& r)
     hipLaunchKernelGGL(kern, dim3((unsigned)nblk), dim3(64 * waves), lds,
                        r.stream, (const T*)r.tab, (const T*)r.x, (T*)r.y,
                        (const T*)r.G, (const T*)r.detJw, r.dofmap,
-                       (long long)r.num_cells, r.skip, ba);
+                       (long long)r.num_cells, r.skip, ba, nullptr);
     return hip_status(hipGetLastError());
   }
 }

@@ -48,6 +48,10 @@ struct ApplyReq
   // optional device flag: when *skip != 0 at launch time the kernel does
   // nothing (lets a solver queue iterations past convergence for free)
   const int* skip = nullptr;
+  // optional: per-wave partial sums of x^T A x (the quadrature-point
+  // energy), one double per wave, [ceil(C/CPB)][waves per workgroup];
+  // unblocked launches only
+  double* energy = nullptr;
   hipStream_t stream = nullptr;
   // LDS-assembled scatter (gather only).  Cells are taken as numbered
   // (ci*ncy + cj)*ncz + ck on an ncx x ncy x ncz grid and processed in

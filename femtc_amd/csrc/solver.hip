@@ -15,7 +15,9 @@
 // iterations without reading anything back; a reduction kernel raises
 // `done` on convergence, breakdown or the iteration cap, and every kernel of
 // the iteration (the operator included) returns at once when it is set.  The
-// host checks `done` once per batch.
+// host checks `done` once per batch.  p.Ap is not a separate pass over p and
+// Ap: the operator kernel returns per-wave partials of the quadrature-point
+// energy sum_q (d.S d + detJw u^2) = sum_e p_e^T A_e p_e, which is p^T A p.
 #include <algorithm>
 #include <vector>
 #include <chrono>
@@ -158,18 +160,21 @@ __global__ void __launch_bounds__(RB)
   }
 }
 
-template <typename V>
+// First stage of p.Ap: the operator kernel's per-wave quadrature-point
+// energies p_e^T A_e p_e (npart of them) summed into NBLK block partials.
 __global__ void __launch_bounds__(RB)
-    k_dot(const V* __restrict__ p, const V* __restrict__ Ap, int64_t n,
-          double* __restrict__ part, const PcgState* __restrict__ st)
+    k_energy_partials(const double* __restrict__ e, int64_t npart,
+                      double* __restrict__ part,
+                      const PcgState* __restrict__ st)
 {
   if (st->done)
     return;
   double s[1] = {0.0};
-  GRID_LOOP(i, n) { s[0] += (double)p[i] * (double)Ap[i]; }
+  GRID_LOOP(i, npart) { s[0] += e[i]; }
   store_partials<1>(s, part);
 }
 
+// alpha = r.z / p.Ap (block partials combined in a fixed order)
 __global__ void __launch_bounds__(RB)
     k_alpha(const double* __restrict__ part, PcgState* st)
 {
@@ -517,6 +522,8 @@ struct femtc_solver
   bool outer_affine = false;
   double* tabw64 = nullptr; // [phi | dphi | w], FP64
   double* cg64 = nullptr;   // [C][7], FP64
+  double* epart = nullptr;  // per-wave p.Ap partials of the inner operator
+  int64_t nepart = 0;
   int check_every = 16;
 };
 
@@ -577,6 +584,7 @@ void destroy(femtc_solver* S)
   }
   dev_free(S->tabw64);
   dev_free(S->cg64);
+  dev_free(S->epart);
   delete S;
 }
 
@@ -667,6 +675,7 @@ int pcg_run(femtc_solver* S, double rtol, int maxit, int* iters, int* status)
   LAUNCH_CHECK();
   ApplyReq a = inner_req(S, dtype_of<V>(), p, Ap);
   a.skip = &S->st->done;
+  a.energy = S->epart; // p.Ap from the operator kernel itself
   for (;;)
   {
     for (int k = 0; k < S->check_every; ++k)
@@ -674,8 +683,9 @@ int pcg_run(femtc_solver* S, double rtol, int maxit, int* iters, int* status)
       int st = run(a);
       if (st != FEMTC_OK)
         return st;
-      hipLaunchKernelGGL(k_dot<V>, dim3(NBLK), dim3(RB), 0, s, (const V*)p,
-                         (const V*)Ap, n, S->part, (const PcgState*)S->st);
+      hipLaunchKernelGGL(k_energy_partials, dim3(NBLK), dim3(RB), 0, s,
+                         (const double*)S->epart, S->nepart, S->part,
+                         (const PcgState*)S->st);
       hipLaunchKernelGGL(k_alpha, dim3(1), dim3(RB), 0, s,
                          (const double*)S->part, S->st);
       hipLaunchKernelGGL(k_update<V>, dim3(NBLK), dim3(RB), 0, s, e,
@@ -828,6 +838,15 @@ int femtc_solver_create(femtc_solver** out, const femtc_problem* prob,
         (st = dev_alloc(&S->rhs, vs * n)) || (st = dev_alloc(&S->e, vs * n)) ||
         (st = dev_alloc(&S->p, vs * n)) || (st = dev_alloc(&S->Ap, vs * n)))
       break;
+    {
+      // one energy partial per wave of the operator launch
+      const int64_t Qq = (int64_t)S->Q * S->Q;
+      const int64_t cpw = Qq <= 64 ? 64 / Qq : 1;
+      const int64_t waves = (cpw * Qq + 63) / 64;
+      S->nepart = (C + cpw - 1) / cpw * waves;
+      if ((st = dev_alloc((void**)&S->epart, 8 * S->nepart)))
+        break;
+    }
     if (hipHostMalloc((void**)&S->hst, sizeof(PcgState)) != hipSuccess ||
         hipHostMalloc((void**)&S->hscal, sizeof(double)) != hipSuccess)
     {
