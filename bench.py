@@ -21,7 +21,7 @@ Beside the metric line's required fields it reports, on the same GPU:
              cores, on a bounded sample of cells of the same shape
   solve      (K+M)u = f on the same mesh to 1e-10: FP64 Jacobi-PCG vs FP32
              inner PCG with FP64 defect correction (solves/s, error)
-  config2    the P=4 64^3 stiffness apply (GDoF/s, roofline fraction)
+  apply_p4_64cube  the P=4 64^3 stiffness apply (GDoF/s, roofline fraction)
   batch      200 uneven meshes in one batched, graph-captured launch set
   serving    a 500 req/s Poisson stream of applies and solves (p50/p99)
 
@@ -237,7 +237,7 @@ def bench_solve(torch, F, n=48, P=6):
     return out
 
 
-def bench_config2(torch, F, steps=30, warmup=60):
+def bench_apply_p4(torch, F, steps=30, warmup=60):
     pr = structured(torch, F, 64, 4)
     x = torch.from_numpy(_uniform(4, pr["ng"], -1, 1)).cuda()
     y = torch.empty(pr["ng"], device="cuda", dtype=torch.float64)
@@ -305,13 +305,13 @@ def main():
     ap.add_argument("--degree", type=int, default=6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--apply-only", action="store_true",
-                    help="skip the solve, config2, batch and serving parts")
+                    help="skip the solve, P=4 apply, batch and serving parts")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
                     help="only copy MiB three times (counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {a.gpus} is not supported "
-              "(the cell loop shards by contiguous cell ranges, see DESIGN.md)",
+              "(one process per GPU would own a contiguous range of cells)",
               file=sys.stderr)
         sys.exit(2)
 
@@ -381,7 +381,7 @@ def main():
     del x, y, pr
     torch.cuda.empty_cache()
     if not a.apply_only:
-        for key, fn in (("solve", bench_solve), ("config2", bench_config2),
+        for key, fn in (("solve", bench_solve), ("apply_p4_64cube", bench_apply_p4),
                         ("batch", bench_batch), ("serving", bench_serving)):
             print(f"[bench] {key}", file=sys.stderr, flush=True)
             out[key] = fn(torch, F)

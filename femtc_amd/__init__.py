@@ -5,7 +5,8 @@ libfemtc.so (include/femtc.h).  Importing the package does not load the
 library; the first call does, and fails loudly if it is not built.
 """
 from ._lib import (BF16, F32, F64, MODE_CORRECT, MODE_REFERENCE, OP_KPM,
-                   OP_MASS, OP_STIFFNESS, FemtcError, lib)
+                   OP_MASS, OP_STIFFNESS, SCATTER_ASSEMBLED, SCATTER_AUTO,
+                   SCATTER_DIRECT, FemtcError, lib)
 from .linalg import Order, string2order
 from .mesh import (basis_tables, basis_tables_weights, dofmap_structured,
                    gauss_rule, geometry_affine, geometry_cell, gll_points)
@@ -21,7 +22,8 @@ __all__ = [
     "Batch", "Plan", "PCGSolver", "ResidentProblem", "Server",
     "global_diagonal", "latency_stats",
     "BF16", "F32", "F64", "MODE_CORRECT", "MODE_REFERENCE", "OP_KPM", "OP_MASS",
-    "OP_STIFFNESS", "FemtcError", "lib", "Order", "string2order", "fill_zero",
+    "OP_STIFFNESS", "SCATTER_ASSEMBLED", "SCATTER_AUTO", "SCATTER_DIRECT",
+    "FemtcError", "lib", "Order", "string2order", "fill_zero",
     "basis_tables", "dofmap_structured", "gauss_rule", "geometry_affine",
     "gll_points", "global_apply", "kpm_apply", "mass_apply", "mass_operator",
     "stiffness_apply", "stiffness_operator",

@@ -101,6 +101,7 @@ SIGNATURES = {
     "femtc_solver_solve": (_i32, [_vp, _vp, _vp, C.c_double, C.c_double, _i32,
                                   _vp]),
     "femtc_solver_diagonal": (_i32, [_vp, C.POINTER(_vp)]),
+    "femtc_solver_scatter": (_i32, [_vp, C.POINTER(_i32)]),
     "femtc_solver_destroy": (_i32, [_vp]),
 }
 
@@ -109,7 +110,11 @@ class Problem(C.Structure):
     """femtc_problem (include/femtc.h)."""
     _fields_ = [("op", _i32), ("degree", _i32), ("nq", _i32),
                 ("num_cells", _i64), ("ndofs", _i64), ("dofmap", _vp),
-                ("tables", _vp), ("G", _vp), ("detJw", _vp), ("J", _vp)]
+                ("tables", _vp), ("G", _vp), ("detJw", _vp), ("J", _vp),
+                ("grid", _i32 * 3), ("scatter", _i32)]
+
+
+SCATTER_AUTO, SCATTER_DIRECT, SCATTER_ASSEMBLED = 0, 1, 2
 
 
 class BatchItem(C.Structure):

@@ -56,6 +56,7 @@
 // 2 N Q table (which does not fit and would spill).
 #pragma once
 
+#include "assemble.hpp"
 #include "common.hpp"
 #include "lds_layout.hpp"
 
@@ -216,6 +217,10 @@ inline size_t blocked_box_bytes(size_t box)
 // BW = 0: CPB consecutive cells, direct scatter.
 // BW > 0: blocked scatter, BW waves per workgroup and one block of
 // cbx x cby x cbz cells per workgroup (Q <= 8 and cbz % CPB == 0).
+// BW = SCATTER_ASSEMBLED: the cell pass of the assembled scatter; every
+// cell-dof value is stored (no atomics) into the lattice-ordered buffer y of
+// lattice_index(), and a second pass (assemble.hpp) sums the values of each
+// global dof.  Cells are numbered (ci*ncy + cj)*ncz + ck.
 template <typename T, int N, int Q, int OP, bool REF, bool GATHER, int BW>
 __device__ __forceinline__ void
 apply_body(const T* __restrict__ tab, const T* __restrict__ x,
@@ -225,7 +230,9 @@ apply_body(const T* __restrict__ tab, const T* __restrict__ x,
            double* __restrict__ epart = nullptr)
 {
   constexpr bool BLK = BW > 0;
+  constexpr bool ASM = BW == SCATTER_ASSEMBLED;
   static_assert(!BLK || (GATHER && Q * Q <= 64), "blocked scatter layout");
+  static_assert(!ASM || (GATHER && !REF), "assembled scatter");
   constexpr int CPB = cells_per_block<Q>();
   constexpr int BT = BLK ? 64 : block_threads<Q>(); // threads per cell group
   constexpr int NL = Q * Q;
@@ -358,10 +365,26 @@ apply_body(const T* __restrict__ tab, const T* __restrict__ x,
     int32_t idx[GATHER ? N : 1];
     if (live && role1)
     {
+      // assembled: the dofmap is the structured numbering (verified by the
+      // caller), so the gather indices are computed, not loaded
+      long long sbase = 0, sstep = 0;
+      if constexpr (ASM)
+      {
+        const long long ck = cell % ba.ncz;
+        const long long t = cell / ba.ncz;
+        const long long cj = t % ba.ncy, ci = t / ba.ncy;
+        constexpr int P1 = N - 1;
+        sstep = (long long)ba.ncz * P1 + 1;
+        sbase = ((ci * P1 + r1a) * ((long long)ba.ncy * P1 + 1) + cj * P1) *
+                    sstep +
+                ck * P1 + r1c;
+      }
 #pragma unroll
       for (int b = 0; b < N; ++b)
       {
-        if constexpr (GATHER)
+        if constexpr (ASM)
+          u[b] = x[sbase + b * sstep];
+        else if constexpr (GATHER)
         {
           idx[b] = dofmap[cell * N3 + uoff + b * N];
           u[b] = x[idx[b]];
@@ -742,6 +765,16 @@ apply_body(const T* __restrict__ tab, const T* __restrict__ x,
         if constexpr (FULL)
           sb[q] = Xc[X_SB * SX + q * XQ + r1c * XC + r1a];
       }
+      // assembled: lattice position of (a, 0, c) of this thread's cell
+      long long wpos = 0, wrow = 0;
+      if constexpr (ASM)
+      {
+        const long long ck = cell % ba.ncz;
+        const long long t = cell / ba.ncz;
+        const long long cj = t % ba.ncy, ci = t / ba.ncy;
+        wrow = (long long)ba.ncz * N;
+        wpos = ((ci * N + r1a) * ba.ncy + cj) * N * wrow + ck * N + r1c;
+      }
 #pragma unroll
       for (int b = 0; b < N; ++b)
       {
@@ -768,6 +801,8 @@ apply_body(const T* __restrict__ tab, const T* __restrict__ x,
           else
             atomic_add(y + idx[b], w);
         }
+        else if constexpr (ASM)
+          y[wpos + b * wrow] = w;
         else if constexpr (GATHER)
           atomic_add(y + idx[b], w);
         else
@@ -820,7 +855,7 @@ constexpr int min_waves_per_simd()
 }
 
 template <typename T, int N, int Q, int OP, bool REF, bool GATHER, int BW>
-__global__ void __launch_bounds__(BW ? 64 * BW : block_threads<Q>(),
+__global__ void __launch_bounds__(BW > 0 ? 64 * BW : block_threads<Q>(),
                                   (min_waves_per_simd<T, Q, OP, BW>()))
     apply_cells(const T* __restrict__ tab, const T* __restrict__ x,
                 T* __restrict__ y, const T* __restrict__ G,
@@ -830,8 +865,20 @@ __global__ void __launch_bounds__(BW ? 64 * BW : block_threads<Q>(),
 {
   if (skip && *skip)
     return;
+  long long blk = blockIdx.x;
+  if constexpr (BW == SCATTER_ASSEMBLED)
+  {
+    // Workgroups are dealt to the 8 XCDs round-robin; give each XCD a
+    // contiguous range of cell blocks so that the partial lines its stores
+    // write meet in its own L2 (a layout choice only: any mapping of
+    // workgroups to blocks is correct).
+    const long long per = (ba.nblk + 7) / 8;
+    blk = (long long)(blockIdx.x % 8) * per + blockIdx.x / 8;
+    if (blk >= ba.nblk)
+      return; // uniform over the workgroup
+  }
   apply_body<T, N, Q, OP, REF, GATHER, BW>(tab, x, y, G, detJw, dofmap,
-                                           num_cells, ba, blockIdx.x, epart);
+                                           num_cells, ba, blk, epart);
 }
 
 // Batched launch: many independent problems of one (T, N, Q, OP, GATHER)
@@ -1007,10 +1054,60 @@ int launch_blocked(const ApplyReq& r)
   }
 }
 
+// Assembled scatter: the cell pass into r.lattice, then (assembled == 1) the
+// assembly pass into y, stream-ordered.
+template <typename T, int N, int Q, int OP>
+int launch_assembled(const ApplyReq& r)
+{
+  constexpr int CPB = cells_per_block<Q>();
+  if (r.ncx < 1 || r.ncy < 1 || r.ncz < 1 || !r.lattice ||
+      (long long)r.ncx * r.ncy * r.ncz != r.num_cells)
+    return FEMTC_ERR_INVALID_ARGUMENT;
+  const long long nblk = (r.num_cells + CPB - 1) / CPB;
+  if (nblk > 0x7ffffff0LL)
+    return FEMTC_ERR_INVALID_ARGUMENT;
+  BlockArgs ba{};
+  ba.ncx = r.ncx;
+  ba.ncy = r.ncy;
+  ba.ncz = r.ncz;
+  ba.nblk = (int)nblk;
+  const long long grid = 8 * ((nblk + 7) / 8);
+  hipLaunchKernelGGL((apply_cells<T, N, Q, OP, false, true, SCATTER_ASSEMBLED>),
+                     dim3((unsigned)grid), dim3(block_threads<Q>()), 0,
+                     r.stream, (const T*)r.tab, (const T*)r.x, (T*)r.lattice,
+                     (const T*)r.G, (const T*)r.detJw, r.dofmap,
+                     (long long)r.num_cells, r.skip, ba, r.energy);
+  if (r.assembled == 1)
+  {
+    const int st = hip_status(hipGetLastError());
+    if (st != FEMTC_OK)
+      return st;
+    return launch_assemble<T, N - 1>(
+        (const T*)r.lattice, (T*)r.y, LatticeGrid{r.ncx, r.ncy, r.ncz, N - 1},
+        r.overwrite ? 0 : 1, r.skip, r.stream);
+  }
+  return hip_status(hipGetLastError());
+}
+
 template <typename T, int N, int Q>
 int dispatch_op(const ApplyReq& r)
 {
   const bool g = r.gather != 0;
+  if (r.assembled)
+  {
+    if (!g || r.reference_mode || r.blocked)
+      return FEMTC_ERR_INVALID_ARGUMENT;
+    switch (r.op)
+    {
+    case OP_MASS: return launch_assembled<T, N, Q, OP_MASS>(r);
+    case OP_STIFF: return launch_assembled<T, N, Q, OP_STIFF>(r);
+    case OP_KPM: return launch_assembled<T, N, Q, OP_KPM>(r);
+    case OP_MASS_AFF: return launch_assembled<T, N, Q, OP_MASS_AFF>(r);
+    case OP_STIFF_AFF: return launch_assembled<T, N, Q, OP_STIFF_AFF>(r);
+    case OP_KPM_AFF: return launch_assembled<T, N, Q, OP_KPM_AFF>(r);
+    }
+    return FEMTC_ERR_INVALID_ARGUMENT;
+  }
   if (r.blocked)
   {
     if (!g || r.reference_mode)

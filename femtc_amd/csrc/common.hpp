@@ -63,7 +63,17 @@ struct ApplyReq
   int cbx = 0, cby = 0, cbz = 0;
   const uint8_t* touch = nullptr;
   int overwrite = 0; // y was cleared: dofs owned by one block are stored
+  // Assembled scatter (gather only, ncx..ncz as above, the dofmap verified to
+  // be the structured one by verify_structured): the cell pass stores every
+  // cell-dof value into `lattice` (lattice_size() values, assemble.hpp)
+  // without atomics; assembled = 1 then sums them into y (y = sum when
+  // overwrite, else y += sum), assembled = 2 stops after the cell pass.
+  int assembled = 0;
+  void* lattice = nullptr;
 };
+
+// BW template value of the assembled scatter's cell-pass instances.
+constexpr int SCATTER_ASSEMBLED = -1;
 
 // One problem of a batched launch (device-resident descriptor).
 struct BatchDesc
@@ -113,6 +123,9 @@ struct BlockArgs
   int nby, nbz;      // blocks along the two fastest axes
   const uint8_t* touch;
   int overwrite;
+  // assembled scatter: cell blocks; the grid is 8 * ceil(nblk / 8)
+  // workgroups taken XCD-chunked (see apply_cells)
+  int nblk;
 };
 
 using ApplyLauncher = int (*)(const ApplyReq&);
@@ -138,5 +151,11 @@ int check_shape(int degree, int nq, int mode);
 int check_dtype(int dtype);
 int run(const ApplyReq& r); // validated request -> per-degree launcher
 int fill_zero(void* y, int64_t n, int dtype, hipStream_t s);
+
+// *ok = 1 when dofmap [C][N^3] is the structured numbering of the
+// nx x ny x nz cell grid (femtc_dofmap_structured) and ndofs its dof count,
+// which the assembled scatter requires.  Synchronous on `s` (plan.hip).
+int verify_structured(const int32_t* dofmap, int nx, int ny, int nz,
+                      int degree, int64_t ndofs, hipStream_t s, int* ok);
 
 } // namespace femtc

@@ -1,10 +1,14 @@
-// Scatter plans for the continuous operator: LDS-assembled scatter over
-// 3-D blocks of a structured cell numbering, with block shapes per
-// (nq, dtype, op) chosen by default tables or by timing candidates on the
-// caller's own data (femtc_plan_autotune).
+// Scatter plans for the continuous operator over a structured cell
+// numbering: the direct atomic scatter, an LDS-assembled scatter over 3-D
+// blocks of cells, or the assembled scatter (a cell pass storing every
+// cell-dof value without atomics, then a pass summing each global dof;
+// offered when the dofmap is verified to be the structured numbering at
+// plan creation), chosen per (nq, dtype, op) by default tables or by timing
+// the candidates on the caller's own data (femtc_plan_autotune).
 #include <cstdio>
 #include <vector>
 
+#include "assemble.hpp"
 #include "common.hpp"
 
 namespace femtc
@@ -54,6 +58,29 @@ unsigned grid_for(int64_t n)
 {
   const int64_t b = (n + RB - 1) / RB;
   return (unsigned)(b < 4096 ? (b > 0 ? b : 1) : 4096);
+}
+
+// Structured-numbering check: every entry equals the lexicographic global
+// index of its cell's local dof (the numbering of femtc_dofmap_structured).
+__global__ void __launch_bounds__(RB)
+    k_check_structured(const int32_t* __restrict__ dofmap, int ny, int nz,
+                       int P, int64_t n, int* __restrict__ bad)
+{
+  const int N = P + 1;
+  const int64_t NY = (int64_t)ny * P + 1, NZ = (int64_t)nz * P + 1;
+  for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * RB)
+  {
+    const int64_t cell = i / (N * N * N);
+    const int l = (int)(i - cell * N * N * N);
+    const int a = l / (N * N), b = (l / N) % N, c = l % N;
+    const int64_t ck = cell % nz;
+    const int64_t cj = (cell / nz) % ny;
+    const int64_t ci = cell / ((int64_t)nz * ny);
+    const int64_t g = ((ci * P + a) * NY + cj * P + b) * NZ + ck * P + c;
+    if ((int64_t)dofmap[i] != g)
+      bad[0] = 1;
+  }
 }
 
 // cells per one-wave group of the cell kernel (64 / nq^2, nq <= 8)
@@ -115,6 +142,44 @@ Shape default_shape(int degree, int nq, int dtype, int op)
 }
 
 } // namespace
+
+int verify_structured(const int32_t* dofmap, int nx, int ny, int nz,
+                      int degree, int64_t ndofs, hipStream_t s, int* ok)
+{
+  *ok = 0;
+  if (!dofmap || nx < 1 || ny < 1 || nz < 1 || degree < 1 ||
+      degree > FEMTC_MAX_DEGREE)
+    return FEMTC_ERR_INVALID_ARGUMENT;
+  const int P = degree;
+  if (ndofs != ((int64_t)nx * P + 1) * ((int64_t)ny * P + 1) *
+                   ((int64_t)nz * P + 1))
+    return FEMTC_OK; // not the structured dof count
+  const int n3 = (P + 1) * (P + 1) * (P + 1);
+  const int64_t n = (int64_t)nx * ny * nz * n3;
+  int* bad = nullptr;
+  if (hipMalloc((void**)&bad, sizeof(int)) != hipSuccess)
+  {
+    (void)hipGetLastError();
+    return fail(FEMTC_ERR_OUT_OF_MEMORY, "dofmap check allocation failed");
+  }
+  int hbad = 1;
+  int st = FEMTC_OK;
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(RB), 0, s, (uint32_t*)bad,
+                     (int64_t)1);
+  hipLaunchKernelGGL(k_check_structured, dim3(grid_for(n)), dim3(RB), 0, s,
+                     dofmap, ny, nz, P, n, bad);
+  if (hipGetLastError() != hipSuccess ||
+      hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, s) !=
+          hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    st = fail(FEMTC_ERR_HIP, std::string("dofmap check: ") +
+                                 hipGetErrorString(hipGetLastError()));
+  (void)hipFree(bad);
+  if (st == FEMTC_OK)
+    *ok = hbad ? 0 : 1;
+  return st;
+}
+
 } // namespace femtc
 
 using namespace femtc;
@@ -125,7 +190,13 @@ struct femtc_plan
   int64_t num_cells = 0, ndofs = 0;
   const int32_t* dofmap = nullptr;
   uint8_t* touch = nullptr;
-  // chosen block shape per (nq, dtype, op); x == 0 means direct scatter
+  // the dofmap is the structured numbering (the assembled scatter applies)
+  int structured = 0;
+  // lattice buffers of the assembled scatter, per dtype (allocated when the
+  // scatter is first selected or timed)
+  void* lattice[2] = {nullptr, nullptr};
+  // chosen block shape per (nq, dtype, op); x == 0 means direct scatter,
+  // x < 0 the assembled scatter
   std::vector<std::pair<int, Shape>> shapes;
 
   Shape shape(int nq, int dtype, int op) const
@@ -171,7 +242,16 @@ int plan_request(const femtc_plan* pl, int op, int dtype, int nq,
   r.dofmap = pl->dofmap;
   r.num_cells = pl->num_cells;
   r.stream = st;
-  if (s.x > 0)
+  if (s.x < 0)
+  {
+    r.assembled = 1;
+    r.lattice = pl->lattice[dtype == FEMTC_F64 ? 0 : 1];
+    r.ncx = pl->nx;
+    r.ncy = pl->ny;
+    r.ncz = pl->nz;
+    r.overwrite = overwrite;
+  }
+  else if (s.x > 0)
   {
     r.blocked = 1;
     r.ncx = pl->nx;
@@ -209,13 +289,37 @@ int check_apply_args(const femtc_plan* pl, int op, int dtype, int nq,
   return FEMTC_OK;
 }
 
+// The assembled scatter's lattice buffer for dtype (allocated once).
+int ensure_lattice(femtc_plan* pl, int dtype)
+{
+  if (!pl->structured)
+    return fail(FEMTC_ERR_INVALID_ARGUMENT,
+                "assembled scatter: the dofmap is not the structured numbering "
+                "of the plan's grid");
+  void*& buf = pl->lattice[dtype == FEMTC_F64 ? 0 : 1];
+  if (buf)
+    return FEMTC_OK;
+  const size_t bytes = (size_t)lattice_size(pl->nx, pl->ny, pl->nz,
+                                            pl->degree) *
+                       (dtype == FEMTC_F64 ? 8 : 4);
+  if (hipMalloc(&buf, bytes) != hipSuccess)
+  {
+    (void)hipGetLastError();
+    buf = nullptr;
+    return fail(FEMTC_ERR_OUT_OF_MEMORY,
+                "assembled scatter buffer allocation failed");
+  }
+  return FEMTC_OK;
+}
+
 int launch_plan(const femtc_plan* pl, int op, int dtype, int nq,
                 const void* tables, const void* x, void* y, const void* G,
                 const void* detJw, int accumulate, Shape s, hipStream_t st)
 {
-  const size_t es = dtype == FEMTC_F64 ? 8 : 4;
-  (void)es;
-  if (!accumulate)
+  if (s.x < 0 && !pl->lattice[dtype == FEMTC_F64 ? 0 : 1])
+    return fail(FEMTC_ERR_INVALID_ARGUMENT,
+                "assembled scatter selected without its buffer");
+  if (!accumulate && s.x >= 0) // the assembly pass overwrites by itself
   {
     int r0 = fill_zero(y, pl->ndofs, dtype, st);
     if (r0 != FEMTC_OK)
@@ -287,6 +391,9 @@ int femtc_plan_create(femtc_plan** out, int nx, int ny, int nz, int degree,
       st = fail(FEMTC_ERR_INVALID_ARGUMENT,
                 "dofmap holds an index outside [0, ndofs)");
   }
+  if (st == FEMTC_OK)
+    st = verify_structured(dofmap, nx, ny, nz, degree, ndofs, s,
+                           &pl->structured);
   if (cnt)
     (void)hipFree(cnt);
   if (bad)
@@ -308,6 +415,9 @@ int femtc_plan_destroy(femtc_plan* plan)
   {
     if (plan->touch)
       (void)hipFree(plan->touch);
+    for (void* b : plan->lattice)
+      if (b)
+        (void)hipFree(b);
     delete plan;
   }
   return FEMTC_OK;
@@ -321,6 +431,16 @@ int femtc_plan_set_block(femtc_plan* plan, int nq, int dtype, int op,
   if (cbx == 0 && cby == 0 && cbz == 0)
   {
     plan->set(nq, dtype, op, Shape{0, 0, 0});
+    return FEMTC_OK;
+  }
+  if (cbx == -1 && cby == -1 && cbz == -1)
+  {
+    int st = check_dtype(dtype);
+    if (st == FEMTC_OK)
+      st = ensure_lattice(plan, dtype);
+    if (st != FEMTC_OK)
+      return st;
+    plan->set(nq, dtype, op, Shape{-1, -1, -1});
     return FEMTC_OK;
   }
   const int cpw = cells_per_wave(nq);
@@ -371,6 +491,8 @@ int femtc_plan_autotune(femtc_plan* plan, int op, int dtype, int nq,
   hipStream_t s = (hipStream_t)stream;
   std::vector<Shape> cand = candidates(nq);
   cand.push_back(Shape{0, 0, 0}); // direct scatter
+  if (plan->structured && ensure_lattice(plan, dtype) == FEMTC_OK)
+    cand.push_back(Shape{-1, -1, -1}); // assembled scatter
   hipEvent_t e0, e1;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
     return fail(FEMTC_ERR_HIP, "event creation failed");

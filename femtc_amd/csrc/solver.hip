@@ -18,12 +18,16 @@
 // host checks `done` once per batch.  p.Ap is not a separate pass over p and
 // Ap: the operator kernel returns per-wave partials of the quadrature-point
 // energy sum_q (d.S d + detJw u^2) = sum_e p_e^T A_e p_e, which is p^T A p.
+// With the assembled scatter (structured dofmap) the operator's cell pass
+// stores its cell values without atomics and the CG update kernel sums them
+// per dof itself, so Ap is never stored.
 #include <algorithm>
 #include <vector>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 
+#include "assemble.hpp"
 #include "common.hpp"
 
 namespace femtc
@@ -39,6 +43,9 @@ struct PcgState
 {
   double rz, rr, pAp, alpha, beta, tol2, rr0;
   int it, maxit, done, status;
+  // alpha of this iteration is valid and e += alpha p is still to be
+  // applied (by k_pupdate, which reads p anyway)
+  int epend;
 };
 
 // Sum of NV values over the block; the result is valid in thread 0.
@@ -179,7 +186,11 @@ __global__ void __launch_bounds__(RB)
     k_alpha(const double* __restrict__ part, PcgState* st)
 {
   if (st->done)
+  {
+    if (threadIdx.x == 0)
+      st->epend = 0; // the last step's e += alpha p has been applied
     return;
+  }
   double v[1];
   combine_partials<1>(part, v);
   if (threadIdx.x == 0)
@@ -189,18 +200,22 @@ __global__ void __launch_bounds__(RB)
     {
       st->done = 1;
       st->status = FEMTC_ERR_BREAKDOWN;
+      st->epend = 0;
     }
     else
+    {
       st->alpha = st->rz / v[0];
+      st->epend = 1;
+    }
   }
 }
 
-// e += alpha p, r -= alpha Ap, Ap = 0 (ready for the next scatter);
-// partials of r.z and r.r with z = D^-1 r
+// r -= alpha Ap, Ap = 0 (ready for the next scatter); partials of r.z and
+// r.r with z = D^-1 r (e += alpha p waits for k_pupdate, which reads p)
 template <typename V>
 __global__ void __launch_bounds__(RB)
-    k_update(V* __restrict__ e, const V* __restrict__ p, V* __restrict__ r,
-             V* __restrict__ Ap, const V* __restrict__ dinv, int64_t n,
+    k_update(V* __restrict__ r, V* __restrict__ Ap,
+             const V* __restrict__ dinv, int64_t n,
              const PcgState* __restrict__ st, double* __restrict__ part)
 {
   if (st->done)
@@ -210,12 +225,49 @@ __global__ void __launch_bounds__(RB)
   GRID_LOOP(i, n)
   {
     const V ri = r[i] - a * Ap[i];
-    e[i] += a * p[i];
     r[i] = ri;
     Ap[i] = V(0);
     const V zi = dinv[i] * ri;
     s[0] += (double)ri * (double)zi;
     s[1] += (double)ri * (double)ri;
+  }
+  store_partials<2>(s, part);
+}
+
+// The same update with the assembled scatter: Ap_i is summed from the cell
+// pass's lattice values (no Ap vector, no atomics, nothing to clear).  The
+// fixed grid's waves take the global dof rows in turn (one row per wave,
+// lanes along the row), so the partial sums are still in a fixed order.
+template <typename V, int P>
+__global__ void __launch_bounds__(RB)
+    k_assemble_update(const V* __restrict__ W, V* __restrict__ r,
+                      const V* __restrict__ dinv, LatticeGrid L,
+                      const PcgState* __restrict__ st,
+                      double* __restrict__ part)
+{
+  if (st->done)
+    return;
+  const V a = (V)st->alpha;
+  double s[2] = {0.0, 0.0};
+  const long long nrows = lattice_global_rows(L);
+  const int NZ = L.ncz * P + 1;
+  const int lane = threadIdx.x & 63;
+  for (long long row = (long long)blockIdx.x * (RB / 64) + (threadIdx.x >> 6);
+       row < nrows; row += (long long)NBLK * (RB / 64))
+  {
+    const V* rows[4];
+    const int mask = lattice_rows<V, P>(W, (int)row, L, rows);
+    const long long base = row * NZ;
+#pragma unroll 2
+    for (int gk = lane; gk < NZ; gk += 64)
+    {
+      const long long i = base + gk;
+      const V ri = r[i] - a * lattice_row_sum<V, P>(rows, mask, gk, L.ncz);
+      r[i] = ri;
+      const V zi = dinv[i] * ri;
+      s[0] += (double)ri * (double)zi;
+      s[1] += (double)ri * (double)ri;
+    }
   }
   store_partials<2>(s, part);
 }
@@ -252,17 +304,29 @@ __global__ void __launch_bounds__(RB)
   }
 }
 
-// p = D^-1 r + beta p
+// e += alpha p (this iteration's step), then p = D^-1 r + beta p unless the
+// iteration ended the solve
 template <typename V>
 __global__ void __launch_bounds__(RB)
-    k_pupdate(V* __restrict__ p, const V* __restrict__ r,
+    k_pupdate(V* __restrict__ p, V* __restrict__ e, const V* __restrict__ r,
               const V* __restrict__ dinv, int64_t n,
               const PcgState* __restrict__ st)
 {
-  if (st->done)
+  if (!st->epend)
     return;
+  const V a = (V)st->alpha;
+  if (st->done)
+  {
+    GRID_LOOP(i, n) { e[i] += a * p[i]; }
+    return;
+  }
   const V b = (V)st->beta;
-  GRID_LOOP(i, n) { p[i] = dinv[i] * r[i] + b * p[i]; }
+  GRID_LOOP(i, n)
+  {
+    const V pi = p[i];
+    e[i] += a * pi;
+    p[i] = dinv[i] * r[i] + b * pi;
+  }
 }
 
 // ------------------------------------------------------- outer loop ----
@@ -525,6 +589,10 @@ struct femtc_solver
   double* epart = nullptr;  // per-wave p.Ap partials of the inner operator
   int64_t nepart = 0;
   int check_every = 16;
+  // assembled scatter of the inner operator (problem.grid, structured
+  // dofmap): its lattice buffer in the inner precision
+  bool inner_assembled = false;
+  void* lattice = nullptr;
 };
 
 namespace femtc
@@ -585,6 +653,7 @@ void destroy(femtc_solver* S)
   dev_free(S->tabw64);
   dev_free(S->cg64);
   dev_free(S->epart);
+  dev_free(S->lattice);
   delete S;
 }
 
@@ -622,6 +691,15 @@ ApplyReq inner_req(const femtc_solver* S, int dtype, const void* x, void* y)
     a.op += OP_MASS_AFF; // the same operator with per-cell geometry
     a.detJw = nullptr;
   }
+  if (S->inner_assembled)
+  {
+    // the cell pass only: the CG update kernel assembles
+    a.assembled = 2;
+    a.lattice = S->lattice;
+    a.ncx = S->prob.grid[0];
+    a.ncy = S->prob.grid[1];
+    a.ncz = S->prob.grid[2];
+  }
   return a;
 }
 
@@ -635,6 +713,140 @@ ApplyReq outer_req(const femtc_solver* S, const void* x, void* y)
   }
   return operator_req(S, FEMTC_F64, S->prob.tables, S->prob.G, S->prob.detJw,
                       x, y);
+}
+
+LatticeGrid lattice_grid(const femtc_solver* S)
+{
+  return LatticeGrid{S->prob.grid[0], S->prob.grid[1], S->prob.grid[2],
+                     S->N - 1};
+}
+
+// The vector update of a CG iteration in precision V after the operator:
+// k_update on the atomically scattered Ap, or k_assemble_update on the
+// assembled scatter's cell values.
+template <typename V>
+int launch_update(femtc_solver* S)
+{
+  hipStream_t s = S->stream;
+  if (S->inner_assembled)
+  {
+    const V* W = (const V*)S->lattice;
+    V* r = (V*)S->r;
+    const V* dinv = (const V*)S->dinv;
+    const LatticeGrid L = lattice_grid(S);
+    const PcgState* st = S->st;
+    switch (L.P)
+    {
+#define FEMTC_ASM_UPDATE(PP)                                                  \
+  case PP:                                                                    \
+    hipLaunchKernelGGL((k_assemble_update<V, PP>), dim3(NBLK), dim3(RB), 0,  \
+                       s, W, r, dinv, L, st, S->part);                        \
+    break;
+      FEMTC_ASM_UPDATE(1)
+      FEMTC_ASM_UPDATE(2)
+      FEMTC_ASM_UPDATE(3)
+      FEMTC_ASM_UPDATE(4)
+      FEMTC_ASM_UPDATE(5)
+      FEMTC_ASM_UPDATE(6)
+      FEMTC_ASM_UPDATE(7)
+      FEMTC_ASM_UPDATE(8)
+#undef FEMTC_ASM_UPDATE
+    default:
+      return FEMTC_ERR_DEGREE_NOT_SUPPORTED;
+    }
+  }
+  else
+    hipLaunchKernelGGL(k_update<V>, dim3(NBLK), dim3(RB), 0, s, (V*)S->r,
+                       (V*)S->Ap, (const V*)S->dinv, S->n,
+                       (const PcgState*)S->st, S->part);
+  return hip_status(hipGetLastError());
+}
+
+// Mean time in ms of reps operator-and-update steps (after one warm-up).
+template <typename V>
+int time_step(femtc_solver* S, int reps, float* ms)
+{
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess)
+    return fail(FEMTC_ERR_HIP, "event creation failed");
+  if (hipEventCreate(&e1) != hipSuccess)
+  {
+    (void)hipEventDestroy(e0);
+    return fail(FEMTC_ERR_HIP, "event creation failed");
+  }
+  const ApplyReq a = inner_req(S, dtype_of<V>(), S->p, S->Ap);
+  int st = FEMTC_OK;
+  for (int i = 0; i <= reps && st == FEMTC_OK; ++i)
+  {
+    if (i == 1 && hipEventRecord(e0, S->stream) != hipSuccess)
+      st = FEMTC_ERR_HIP;
+    if (st == FEMTC_OK)
+      st = run(a);
+    if (st == FEMTC_OK)
+      st = launch_update<V>(S);
+  }
+  if (st == FEMTC_OK && (hipEventRecord(e1, S->stream) != hipSuccess ||
+                         hipEventSynchronize(e1) != hipSuccess ||
+                         hipEventElapsedTime(ms, e0, e1) != hipSuccess))
+    st = FEMTC_ERR_HIP;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (st == FEMTC_ERR_HIP)
+    return fail(st, std::string("solver scatter timing: ") +
+                        hipGetErrorString(hipGetLastError()));
+  *ms /= reps;
+  return st;
+}
+
+// Scatter of the inner operator from problem.grid/scatter: the assembled
+// one when asked for, or, under FEMTC_SCATTER_AUTO, when its iteration step
+// times faster than the atomic one on this problem (vectors zeroed and
+// alpha = 0 for the timing; the solve re-initialises everything).
+int choose_scatter(femtc_solver* S, int structured)
+{
+  const int mode = S->prob.scatter;
+  if (!structured || mode == FEMTC_SCATTER_DIRECT)
+    return FEMTC_OK;
+  const size_t es = S->inner_dtype == FEMTC_F64 ? 8 : 4;
+  if (dev_alloc(&S->lattice,
+                es * (size_t)lattice_size(S->prob.grid[0], S->prob.grid[1],
+                                          S->prob.grid[2], S->N - 1)))
+  {
+    if (mode == FEMTC_SCATTER_ASSEMBLED)
+      return FEMTC_ERR_OUT_OF_MEMORY;
+    return FEMTC_OK; // no room: keep the atomic scatter
+  }
+  if (mode == FEMTC_SCATTER_ASSEMBLED)
+  {
+    S->inner_assembled = true;
+    return FEMTC_OK;
+  }
+  hipStream_t s = S->stream;
+  const int dt = S->inner_dtype;
+  int st = FEMTC_OK;
+  for (void* v : {S->e, S->r, S->p, S->Ap})
+    if (st == FEMTC_OK)
+      st = fill_zero(v, S->n, dt, s);
+  PcgState h{};
+  if (st == FEMTC_OK &&
+      (hipMemcpyAsync(S->st, &h, sizeof(h), hipMemcpyHostToDevice, s) !=
+           hipSuccess ||
+       hipStreamSynchronize(s) != hipSuccess))
+    st = fail(FEMTC_ERR_HIP, "solver: state upload failed");
+  float t[2] = {0.f, 0.f};
+  for (int c = 0; c < 2 && st == FEMTC_OK; ++c)
+  {
+    S->inner_assembled = c == 1;
+    st = dt == FEMTC_F64 ? time_step<double>(S, 3, &t[c])
+                         : time_step<float>(S, 3, &t[c]);
+  }
+  S->inner_assembled = st == FEMTC_OK && t[1] < t[0];
+  if (!S->inner_assembled)
+  {
+    dev_free(S->lattice);
+    S->lattice = nullptr;
+  }
+  return st;
 }
 
 int sync_read(femtc_solver* S, void* host, const void* dev, size_t bytes)
@@ -688,12 +900,12 @@ int pcg_run(femtc_solver* S, double rtol, int maxit, int* iters, int* status)
                          (const PcgState*)S->st);
       hipLaunchKernelGGL(k_alpha, dim3(1), dim3(RB), 0, s,
                          (const double*)S->part, S->st);
-      hipLaunchKernelGGL(k_update<V>, dim3(NBLK), dim3(RB), 0, s, e,
-                         (const V*)p, r, Ap, dinv, n, (const PcgState*)S->st,
-                         S->part);
+      if ((st = launch_update<V>(S)) != FEMTC_OK)
+        return fail(st, std::string("solver kernel launch: ") +
+                            hipGetErrorString(hipGetLastError()));
       hipLaunchKernelGGL(k_beta, dim3(1), dim3(RB), 0, s,
                          (const double*)S->part, S->st);
-      hipLaunchKernelGGL(k_pupdate<V>, dim3(NBLK), dim3(RB), 0, s, p,
+      hipLaunchKernelGGL(k_pupdate<V>, dim3(NBLK), dim3(RB), 0, s, p, e,
                          (const V*)r, dinv, n, (const PcgState*)S->st);
       LAUNCH_CHECK();
     }
@@ -765,6 +977,18 @@ int check_problem(const femtc_problem* P)
   if (P->op != FEMTC_OP_STIFFNESS && !P->detJw && !P->J)
     return fail(FEMTC_ERR_INVALID_ARGUMENT,
                 "detJw (or the cell Jacobians J) is required for this op");
+  const bool nogrid = P->grid[0] == 0 && P->grid[1] == 0 && P->grid[2] == 0;
+  if (!nogrid && (P->grid[0] < 1 || P->grid[1] < 1 || P->grid[2] < 1 ||
+                  (int64_t)P->grid[0] * P->grid[1] * P->grid[2] !=
+                      P->num_cells))
+    return fail(FEMTC_ERR_INVALID_ARGUMENT,
+                "grid must be all zero or positive with grid[0]*grid[1]*grid[2] "
+                "== num_cells");
+  if (P->scatter < FEMTC_SCATTER_AUTO || P->scatter > FEMTC_SCATTER_ASSEMBLED)
+    return fail(FEMTC_ERR_INVALID_ARGUMENT, "scatter must be a femtc_scatter");
+  if (P->scatter == FEMTC_SCATTER_ASSEMBLED && nogrid)
+    return fail(FEMTC_ERR_INVALID_ARGUMENT,
+                "the assembled scatter needs the structured grid");
   return FEMTC_OK;
 }
 
@@ -839,7 +1063,8 @@ int femtc_solver_create(femtc_solver** out, const femtc_problem* prob,
         (st = dev_alloc(&S->p, vs * n)) || (st = dev_alloc(&S->Ap, vs * n)))
       break;
     {
-      // one energy partial per wave of the operator launch
+      // one energy partial per wave of the operator launch (the same cell
+      // blocks under either scatter)
       const int64_t Qq = (int64_t)S->Q * S->Q;
       const int64_t cpw = Qq <= 64 ? 64 / Qq : 1;
       const int64_t waves = (cpw * Qq + 63) / 64;
@@ -957,6 +1182,22 @@ int femtc_solver_create(femtc_solver** out, const femtc_problem* prob,
                 "no cell touches, or geometry that is not positive definite)");
       break;
     }
+    // scatter of the inner operator
+    int structured = 0;
+    if (prob->grid[0] > 0 &&
+        (st = verify_structured(prob->dofmap, prob->grid[0], prob->grid[1],
+                                prob->grid[2], prob->degree, n, s,
+                                &structured)) != FEMTC_OK)
+      break;
+    if (prob->scatter == FEMTC_SCATTER_ASSEMBLED && !structured)
+    {
+      st = fail(FEMTC_ERR_INVALID_ARGUMENT,
+                "assembled scatter: the dofmap is not the structured "
+                "numbering of the grid");
+      break;
+    }
+    if ((st = choose_scatter(S, structured)) != FEMTC_OK)
+      break;
   } while (false);
   if (st != FEMTC_OK)
   {
@@ -978,6 +1219,15 @@ int femtc_solver_diagonal(const femtc_solver* solver, const double** diag)
   if (!solver || !diag)
     return fail(FEMTC_ERR_INVALID_ARGUMENT, "NULL argument");
   *diag = solver->diag;
+  return FEMTC_OK;
+}
+
+int femtc_solver_scatter(const femtc_solver* solver, int* scatter)
+{
+  if (!solver || !scatter)
+    return fail(FEMTC_ERR_INVALID_ARGUMENT, "NULL argument");
+  *scatter = solver->inner_assembled ? FEMTC_SCATTER_ASSEMBLED
+                                     : FEMTC_SCATTER_DIRECT;
   return FEMTC_OK;
 }
 

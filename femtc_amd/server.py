@@ -19,12 +19,14 @@ class ResidentProblem:
     detJw (FP64 torch tensors), num_cells, ndofs."""
 
     def __init__(self, op, degree, nq, tables, dofmap, num_cells, ndofs,
-                 G=None, detJw=None, J=None):
+                 G=None, detJw=None, J=None, grid=None):
         self._keep = (tables, dofmap, G, detJw, J)
         self.ndofs = ndofs
+        g = (0, 0, 0) if grid is None else tuple(int(v) for v in grid)
         self.struct = _lib.Problem(op, degree, nq, num_cells, ndofs,
                                    _ptr(dofmap), _ptr(tables), _ptr(G),
-                                   _ptr(detJw), _ptr(J))
+                                   _ptr(detJw), _ptr(J), (C.c_int32 * 3)(*g),
+                                   _lib.SCATTER_AUTO)
 
     @classmethod
     def from_item(cls, it, affine: bool = False):

@@ -32,20 +32,35 @@ class PCGSolver:
 
     def __init__(self, op: int, tables, dofmap, num_cells: int, degree: int,
                  nq: int, ndofs: int, G=None, detJw=None, inner: int = _lib.F64,
-                 stream=None, J=None):
+                 stream=None, J=None, grid=None,
+                 scatter: int = _lib.SCATTER_AUTO):
         """J: optional per-cell Jacobians [C*9] (FP64) of affine cells; the
-        inner operator then computes its geometry in the kernel."""
+        inner operator then computes its geometry in the kernel.
+        grid: optional (nx, ny, nz) of a structured cell numbering; when the
+        dofmap is that numbering the CG iterations can use the assembled
+        scatter (scatter: SCATTER_AUTO times it against the atomic one,
+        SCATTER_DIRECT / SCATTER_ASSEMBLED force one)."""
         self._keep = (tables, dofmap, G, detJw, J)
         self.ndofs = ndofs
         self.inner = inner
+        g = (0, 0, 0) if grid is None else tuple(int(v) for v in grid)
         self.problem = _lib.Problem(op, degree, nq, num_cells, ndofs,
                                     _ptr(dofmap), _ptr(tables), _ptr(G),
-                                    _ptr(detJw), _ptr(J))
+                                    _ptr(detJw), _ptr(J), (C.c_int32 * 3)(*g),
+                                    scatter)
         h = C.c_void_p()
         _lib.check(_lib.lib().femtc_solver_create(
             C.byref(h), C.byref(self.problem), inner, _stream(stream)),
             "solver_create")
         self._h = h
+
+    def scatter(self) -> int:
+        """Scatter of the CG iterations chosen at creation: SCATTER_DIRECT or
+        SCATTER_ASSEMBLED."""
+        a = C.c_int32()
+        _lib.check(_lib.lib().femtc_solver_scatter(self._h, C.byref(a)),
+                   "solver_scatter")
+        return a.value
 
     def solve(self, f, u, rtol: float = 1e-10, inner_rtol: float = 1e-4,
               maxit: int = 20000, check: bool = True) -> dict:

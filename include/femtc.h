@@ -222,14 +222,23 @@ int femtc_geometry_affine(int64_t num_cells, int nq, const double* J,
 /* Results are correct for any dofmap; the blocking pays off when the      */
 /* numbering is the structured one.  The plan keeps a pointer to dofmap    */
 /* (which must stay alive) and 1 byte per dof of touch counts.             */
+/* Plan creation also checks whether the dofmap is exactly the structured  */
+/* numbering (femtc_dofmap_structured); then the assembled scatter is      */
+/* available: a cell pass stores every cell-dof value without atomics into */
+/* a plan-owned buffer of C*(P+1)^3 values (allocated when the scatter is  */
+/* first selected or timed), and a second pass sums the values of each     */
+/* global dof into y.  Applies of one plan through it share that buffer:   */
+/* they must not overlap in time (e.g. on two streams).                    */
 /* ---------------------------------------------------------------------- */
 typedef struct femtc_plan femtc_plan; /* opaque */
 int femtc_plan_create(femtc_plan** plan, int nx, int ny, int nz, int degree,
                       const int32_t* dofmap, int64_t ndofs, void* stream);
 int femtc_plan_destroy(femtc_plan* plan);
 /* Block shape (cells per axis) for one (nq, dtype, op); 0,0,0 selects the
- * direct atomic scatter.  cbz must be a multiple of 64/nq^2 and the block
- * at most 8 waves of 64/nq^2 cells; nq <= 8. */
+ * direct atomic scatter and -1,-1,-1 the assembled scatter (refused with
+ * FEMTC_ERR_INVALID_ARGUMENT when the dofmap is not the structured one).
+ * cbz must be a multiple of 64/nq^2 and the block at most 8 waves of
+ * 64/nq^2 cells; nq <= 8. */
 int femtc_plan_set_block(femtc_plan* plan, int nq, int dtype, int op, int cbx,
                          int cby, int cbz);
 int femtc_plan_get_block(const femtc_plan* plan, int nq, int dtype, int op,
@@ -239,8 +248,9 @@ int femtc_plan_apply(const femtc_plan* plan, int op, int dtype, int nq,
                      const void* tables, const void* x, void* y,
                      const void* G, const void* detJw, int accumulate,
                      void* stream);
-/* Time every candidate block shape (and the direct scatter) on these
- * buffers, reps applies each, and keep the fastest for (nq, dtype, op).
+/* Time every candidate block shape (and the direct and, for the structured
+ * dofmap, the assembled scatter) on these buffers, reps applies each, and
+ * keep the fastest for (nq, dtype, op).
  * y is left cleared.  best_us (may be NULL) receives its time per apply. */
 int femtc_plan_autotune(femtc_plan* plan, int op, int dtype, int nq,
                         const void* tables, const void* x, void* y,
@@ -328,7 +338,28 @@ typedef struct femtc_problem
    * detJw; G/detJw may then be NULL, and the FP64 residual uses the
    * per-cell geometry too. */
   const double* J;
+  /* Optional structured cell numbering: cell (i, j, k) of an
+   * grid[0] x grid[1] x grid[2] grid is (i*grid[1] + j)*grid[2] + k; all
+   * zero when unknown.  With a grid the solver checks at creation whether
+   * the dofmap is the structured numbering (femtc_dofmap_structured); then
+   * the CG iteration can run on the assembled scatter (see the plans
+   * above): the operator's cell pass stores its cell values without
+   * atomics into a workspace of C*(P+1)^3 values and the CG update kernel
+   * sums them per dof itself.  `scatter` picks the inner operator's
+   * scatter: */
+  int grid[3];
+  int scatter; /* femtc_scatter */
 } femtc_problem;
+
+typedef enum femtc_scatter
+{
+  FEMTC_SCATTER_AUTO = 0,     /* time one iteration step with each (when the
+                                 dofmap allows it), keep the faster */
+  FEMTC_SCATTER_DIRECT = 1,   /* atomic scatter */
+  FEMTC_SCATTER_ASSEMBLED = 2 /* assembled scatter; creation fails with
+                                 FEMTC_ERR_INVALID_ARGUMENT if the dofmap
+                                 is not the structured one */
+} femtc_scatter;
 
 typedef struct femtc_solve_info
 {
@@ -361,6 +392,9 @@ int femtc_solver_solve(femtc_solver* solver, const double* f, double* u,
                        femtc_solve_info* info);
 /* Device pointer to the solver's FP64 operator diagonal. */
 int femtc_solver_diagonal(const femtc_solver* solver, const double** diag);
+/* The scatter the solver chose for its CG iterations
+ * (FEMTC_SCATTER_DIRECT or FEMTC_SCATTER_ASSEMBLED). */
+int femtc_solver_scatter(const femtc_solver* solver, int* scatter);
 int femtc_solver_destroy(femtc_solver* solver);
 
 /* ---------------------------------------------------------------------- */

@@ -123,6 +123,22 @@ def test_solver_create_rejects_bad_problems_without_touching_the_device():
         _lib.ERR_INVALID_ARGUMENT
     assert L.femtc_solver_create(None, C_.byref(p), F.F64, None) == \
         _lib.ERR_INVALID_ARGUMENT
+    # structured grid and scatter selection
+    grid = C_.c_int32 * 3
+    p = _lib.Problem(F.OP_KPM, 3, 4, 10, 100, 1, 1, 1, 1, None, grid(2, 2, 2))
+    assert L.femtc_solver_create(C_.byref(h), C_.byref(p), F.F64, None) == \
+        _lib.ERR_INVALID_ARGUMENT
+    assert "grid" in _lib.last_error()
+    p = _lib.Problem(F.OP_KPM, 3, 4, 10, 100, 1, 1, 1, 1, None, grid(0, 0, 0),
+                     F.SCATTER_ASSEMBLED)
+    assert L.femtc_solver_create(C_.byref(h), C_.byref(p), F.F64, None) == \
+        _lib.ERR_INVALID_ARGUMENT
+    assert "needs the structured grid" in _lib.last_error()
+    p = _lib.Problem(F.OP_KPM, 3, 4, 10, 100, 1, 1, 1, 1, None, grid(1, 2, 5),
+                     7)
+    assert L.femtc_solver_create(C_.byref(h), C_.byref(p), F.F64, None) == \
+        _lib.ERR_INVALID_ARGUMENT
+    assert "femtc_scatter" in _lib.last_error()
     assert h.value is None
 
 

@@ -152,7 +152,7 @@ def test_host_shim_unsupported_arguments_print_and_leave_w(capfd):
         assert capfd.readouterr().out.strip() == str(msg)
 
 
-def test_config1_mass_full_size():
+def test_mass_cell_local_32768_cells_p3_q5():
     # 32768 cells, P=3 GLL, Q=5 Gauss, U~U[-1,1], detJw~U[0.5,1.5]
     P, Q, C = 3, 5, 32768
     N = P + 1
@@ -211,7 +211,7 @@ def test_global_operator_vs_oracle(op, n, P, Q):
     assert rel(host(dy), ref) <= TOL64
 
 
-def test_config2_stiffness_global_full_size():
+def test_stiffness_global_64cube_p4_q5():
     # 64^3 mesh, P=4, Q=5, J = I + 0.1 U(-1,1) per cell, x ~ U[-1,1]
     n, P, Q = 64, 4, 5
     C, J, G, Dw, dm, ng = _mesh(n, P, Q, 3)
@@ -276,7 +276,7 @@ def test_global_kpm_f32(nx, ny, nz, P, Q):
     assert rel(host(dy), ref) <= TOL32
 
 
-def test_config3_stiffness_global_full_size():
+def test_stiffness_global_48cube_p6_q7():
     # 48^3 mesh, P=6 GLL, Q=7 Gauss: the benchmark workload
     n, P, Q = 48, 6, 7
     C, G, Dw, dm, ng = _box(n, n, n, P, Q, 3)

@@ -1,8 +1,9 @@
 """GPU parity of the planned continuous operator (LDS-assembled blocked
-scatter) against the oracle, through the C ABI: every candidate block shape,
-partial blocks, overwrite and accumulate, FP32, and dofmaps that are not the
-structured numbering (where the plan must fall back to atomics and stay
-correct).  Bar: FP64 <= 1e-12 relative l2, FP32 <= 1e-5."""
+scatter, assembled two-pass scatter) against the oracle, through the C ABI:
+every candidate block shape, partial blocks, overwrite and accumulate, FP32,
+and dofmaps that are not the structured numbering (where the blocked
+scatter must fall back to atomics and stay correct, and the assembled
+scatter is refused).  Bar: FP64 <= 1e-12 relative l2, FP32 <= 1e-5."""
 import numpy as np
 import pytest
 
@@ -114,6 +115,85 @@ def test_plan_on_a_dofmap_that_is_not_structured(kind, P, Q):
     assert rel(host(dy), ref) <= 1e-12
 
 
+# the block meshes plus Q > 8 (two-wave workgroups), a single cell, single
+# rows along each axis and unequal dimensions
+ASSEMBLED_MESHES = MESHES + [(2, 3, 3, 7, 9), (2, 2, 3, 8, 10),
+                             (3, 2, 2, 8, 11), (4, 4, 4, 1, 4), (1, 1, 1, 3, 4),
+                             (1, 1, 5, 2, 3), (5, 1, 1, 2, 4), (1, 4, 1, 3, 4),
+                             (2, 3, 7, 4, 6)]
+
+
+@pytest.mark.parametrize("nx,ny,nz,P,Q", ASSEMBLED_MESHES)
+@pytest.mark.parametrize("op", [F.OP_MASS, F.OP_STIFFNESS, F.OP_KPM])
+def test_plan_assembled_vs_oracle(op, nx, ny, nz, P, Q):
+    C, G, Dw, dm, ng = _box(nx, ny, nz, P, Q, 80 + P)
+    tab = O.basis_tables(P, Q)
+    x = O.uniform(81, ng)
+    ref = O.global_operator(op, P + 1, Q, tab, x, dm, G, Dw)
+    plan = F.Plan(nx, ny, nz, P, dev(dm, torch.int32), ng)
+    dtab, dx, dG, dD = dev(tab), dev(x), dev(G), dev(Dw)
+    plan.set_block(Q, F.F64, op, (-1, -1, -1))
+    assert plan.get_block(Q, F.F64, op) == (-1, -1, -1)
+    dy = torch.full((ng,), 5.0, device="cuda", dtype=torch.float64)
+    plan.apply(op, dtab, dx, dy, Q, G=dG, detJw=dD)
+    assert rel(host(dy), ref) <= 1e-12
+    y0 = O.uniform(82, ng)
+    dy = dev(y0)
+    plan.apply(op, dtab, dx, dy, Q, G=dG, detJw=dD, accumulate=True)
+    assert rel(host(dy), ref + y0) <= 1e-12
+
+
+@pytest.mark.parametrize("P,Q", [(2, 3), (4, 5), (6, 7), (8, 9)])
+def test_plan_assembled_f32(P, Q):
+    nx, ny, nz = 3, 4, 5
+    C, G, Dw, dm, ng = _box(nx, ny, nz, P, Q, 15)
+    tab = O.basis_tables(P, Q)
+    x = O.uniform(16, ng)
+    ref = O.global_operator(F.OP_KPM, P + 1, Q, tab, x, dm, G, Dw)
+    f = torch.float32
+    plan = F.Plan(nx, ny, nz, P, dev(dm, torch.int32), ng)
+    plan.set_block(Q, F.F32, F.OP_KPM, (-1, -1, -1))
+    dy = torch.empty(ng, device="cuda", dtype=f)
+    plan.apply(F.OP_KPM, dev(tab, f), dev(x, f), dy, Q, G=dev(G, f),
+               detJw=dev(Dw, f))
+    assert rel(host(dy), ref) <= 1e-5
+
+
+@pytest.mark.parametrize("kind", ["relabelled dofs", "shuffled cells",
+                                  "one entry changed", "extra dof"])
+def test_plan_assembled_only_for_the_structured_numbering(kind):
+    # any other dofmap is refused for the assembled scatter (and still
+    # applies correctly through the other scatters)
+    nx, ny, nz, P, Q = 3, 4, 4, 2, 3
+    C, G, Dw, dm, ng = _box(nx, ny, nz, P, Q, 17)
+    rng = np.random.default_rng(18)
+    N3 = (P + 1) ** 3
+    if kind == "relabelled dofs":
+        dm = rng.permutation(ng).astype(np.int32)[dm]
+    elif kind == "shuffled cells":
+        cp = rng.permutation(C)
+        dm = dm.reshape(C, N3)[cp].ravel()
+        G = G.reshape(C, -1)[cp].ravel()
+        Dw = Dw.reshape(C, -1)[cp].ravel()
+    elif kind == "one entry changed":
+        dm = dm.copy()
+        dm[2 * N3 + N3 // 2] = dm[N3 // 2]
+    else:
+        ng += 1  # the structured numbering, one unused dof more
+    tab = O.basis_tables(P, Q)
+    x = O.uniform(19, ng)
+    ref = O.global_operator(F.OP_KPM, P + 1, Q, tab, x, dm, G, Dw)
+    plan = F.Plan(nx, ny, nz, P, dev(dm, torch.int32), ng)
+    with pytest.raises(F.FemtcError, match="structured"):
+        plan.set_block(Q, F.F64, F.OP_KPM, (-1, -1, -1))
+    dy = torch.empty(ng, device="cuda", dtype=torch.float64)
+    shape, _ = plan.autotune(F.OP_KPM, dev(tab), dev(x), dy, Q, G=dev(G),
+                             detJw=dev(Dw), reps=1)
+    assert shape != (-1, -1, -1)
+    plan.apply(F.OP_KPM, dev(tab), dev(x), dy, Q, G=dev(G), detJw=dev(Dw))
+    assert rel(host(dy), ref) <= 1e-12
+
+
 def test_plan_autotune_picks_a_valid_shape_and_clears_y():
     nx, ny, nz, P, Q = 8, 8, 8, 4, 5
     C, G, Dw, dm, ng = _box(nx, ny, nz, P, Q, 12)
@@ -126,7 +206,7 @@ def test_plan_autotune_picks_a_valid_shape_and_clears_y():
     shape, us = plan.autotune(F.OP_STIFFNESS, dtab, dx, dy, Q, G=dG, reps=2)
     assert us > 0
     assert float(dy.abs().max()) == 0.0
-    assert shape in _shapes(Q) + [(0, 0, 0)]
+    assert shape in _shapes(Q) + [(0, 0, 0), (-1, -1, -1)]
     plan.apply(F.OP_STIFFNESS, dtab, dx, dy, Q, G=dG)
     assert rel(host(dy), ref) <= 1e-12
 

@@ -139,6 +139,71 @@ def test_refined_f32_reaches_1e10(op, nx, ny, nz, P, Q):
     assert info["outer_iterations"] >= 2  # FP32 alone cannot reach 1e-10
 
 
+ASSEMBLED_SOLVES = [(F.OP_KPM, 3, 3, 4, 4, 5, False),
+                    (F.OP_MASS, 3, 2, 3, 3, 4, False),
+                    (F.OP_KPM, 2, 3, 3, 6, 7, False),
+                    (F.OP_KPM, 3, 2, 3, 4, 5, True),
+                    (F.OP_KPM, 2, 2, 3, 8, 9, True),
+                    (F.OP_KPM, 1, 1, 3, 2, 4, False)]
+
+
+@pytest.mark.parametrize("inner", [F.F64, F.F32])
+@pytest.mark.parametrize("op,nx,ny,nz,P,Q,affine", ASSEMBLED_SOLVES)
+def test_solver_assembled_scatter(inner, op, nx, ny, nz, P, Q, affine):
+    # the same solve with the atomic and the assembled scatter (forced, and
+    # picked by timing): each meets the tolerance under the oracle's
+    # operator and matches the oracle's solution; the assembled update
+    # drives CG like the atomic one (iterations +- 2)
+    seed = 120 + P
+    C, G, Dw, dm, ng, tab, f = _problem(op, nx, ny, nz, P, Q, seed)
+    name = {F.OP_MASS: "mass", F.OP_KPM: "kpm"}[op]
+    u_ref, _, _ = O.pcg(name, P + 1, Q, tab, f, dm, G, Dw, rtol=1e-14)
+    kw = dict(detJw=dev(Dw), G=dev(G) if op != F.OP_MASS else None)
+    if affine:
+        kw["J"] = dev(O.random_jacobians(seed, C))
+    iters = {}
+    for scatter in (F.SCATTER_DIRECT, F.SCATTER_ASSEMBLED, F.SCATTER_AUTO):
+        with F.PCGSolver(op, dev(tab), dev(dm, torch.int32), C, P, Q, ng,
+                         inner=inner, grid=(nx, ny, nz), scatter=scatter,
+                         **kw) as S:
+            kind = S.scatter()
+            if scatter != F.SCATTER_AUTO:
+                assert kind == scatter
+            u = torch.empty(ng, device="cuda", dtype=torch.float64)
+            info = S.solve(dev(f), u, rtol=1e-10)
+            # a second solve on the same solver: the workspace is reused
+            u2 = torch.empty(ng, device="cuda", dtype=torch.float64)
+            S.solve(dev(f), u2, rtol=1e-10)
+        uh = host(u)
+        res = rel(O.global_operator(op, P + 1, Q, tab, uh, dm, G, Dw), f)
+        assert info["relres"] <= 1.01e-10 and res <= 1.05e-10, (scatter, info)
+        assert rel(uh, u_ref) <= 1e-8, scatter
+        assert rel(host(u2), uh) <= 1e-9, scatter
+        iters[scatter] = info["inner_iterations"]
+    assert abs(iters[F.SCATTER_DIRECT] - iters[F.SCATTER_ASSEMBLED]) <= 2, iters
+
+
+def test_solver_assembled_scatter_needs_the_structured_dofmap():
+    nx, ny, nz, P, Q = 3, 2, 3, 2, 3
+    C, G, Dw, dm, ng, tab, f = _problem(F.OP_KPM, nx, ny, nz, P, Q, 7)
+    # the same mesh with its dofs relabelled: a valid dofmap, not the
+    # structured numbering
+    dm = np.random.default_rng(8).permutation(ng).astype(np.int32)[dm]
+    args = (F.OP_KPM, dev(tab), dev(dm, torch.int32), C, P, Q, ng)
+    with pytest.raises(F.FemtcError, match="structured"):
+        F.PCGSolver(*args, G=dev(G), detJw=dev(Dw), grid=(nx, ny, nz),
+                    scatter=F.SCATTER_ASSEMBLED)
+    # the automatic choice keeps the atomic scatter and solves correctly
+    with F.PCGSolver(*args, G=dev(G), detJw=dev(Dw), grid=(nx, ny, nz)) as S:
+        assert S.scatter() == F.SCATTER_DIRECT
+        u = torch.empty(ng, device="cuda", dtype=torch.float64)
+        info = S.solve(dev(f), u, rtol=1e-10)
+    res = rel(O.global_operator("kpm", P + 1, Q, tab, host(u), dm, G, Dw), f)
+    assert info["relres"] <= 1.01e-10 and res <= 1.05e-10
+    with pytest.raises(F.FemtcError, match="grid"):
+        F.PCGSolver(*args, G=dev(G), detJw=dev(Dw), grid=(nx, ny, nz + 1))
+
+
 def test_solver_reuse_is_consistent():
     C, G, Dw, dm, ng, tab, f = _problem(F.OP_KPM, 3, 2, 3, 4, 5, 5)
     with F.PCGSolver(F.OP_KPM, dev(tab), dev(dm, torch.int32), C, 4, 5, ng,
