@@ -102,6 +102,7 @@ SIGNATURES = {
                                   _vp]),
     "femtc_solver_diagonal": (_i32, [_vp, C.POINTER(_vp)]),
     "femtc_solver_scatter": (_i32, [_vp, C.POINTER(_i32)]),
+    "femtc_solver_set_batch": (_i32, [_vp, _i32, _i32]),
     "femtc_solver_destroy": (_i32, [_vp]),
 }
 

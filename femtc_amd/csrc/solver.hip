@@ -48,11 +48,12 @@ struct PcgState
   int epend;
 };
 
-// Sum of NV values over the block; the result is valid in thread 0.
-template <int NV>
+// Sum of NV values over the block of BT threads; the result is valid in
+// thread 0.
+template <int NV, int BT = RB>
 __device__ inline void block_sum(double (&v)[NV])
 {
-  __shared__ double sh[NV][RB / 64];
+  __shared__ double sh[NV][BT / 64];
 #pragma unroll
   for (int k = 0; k < NV; ++k)
   {
@@ -77,17 +78,17 @@ __device__ inline void block_sum(double (&v)[NV])
     for (int k = 0; k < NV; ++k)
     {
       double s = 0.0;
-      for (int j = 0; j < RB / 64; ++j)
+      for (int j = 0; j < BT / 64; ++j)
         s += sh[k][j];
       v[k] = s;
     }
   }
 }
 
-template <int NV>
+template <int NV, int BT = RB>
 __device__ inline void store_partials(double (&v)[NV], double* part)
 {
-  block_sum<NV>(v);
+  block_sum<NV, BT>(v);
   if (threadIdx.x == 0)
   {
 #pragma unroll
@@ -238,8 +239,12 @@ __global__ void __launch_bounds__(RB)
 // pass's lattice values (no Ap vector, no atomics, nothing to clear).  The
 // fixed grid's waves take the global dof rows in turn (one row per wave,
 // lanes along the row), so the partial sums are still in a fixed order.
+// Its workgroups are ARB threads: twice the waves of the other vector
+// kernels, for more loads in flight.
+constexpr int ARB = 512;
+
 template <typename V, int P>
-__global__ void __launch_bounds__(RB)
+__global__ void __launch_bounds__(ARB)
     k_assemble_update(const V* __restrict__ W, V* __restrict__ r,
                       const V* __restrict__ dinv, LatticeGrid L,
                       const PcgState* __restrict__ st,
@@ -252,8 +257,8 @@ __global__ void __launch_bounds__(RB)
   const long long nrows = lattice_global_rows(L);
   const int NZ = L.ncz * P + 1;
   const int lane = threadIdx.x & 63;
-  for (long long row = (long long)blockIdx.x * (RB / 64) + (threadIdx.x >> 6);
-       row < nrows; row += (long long)NBLK * (RB / 64))
+  for (long long row = (long long)blockIdx.x * (ARB / 64) + (threadIdx.x >> 6);
+       row < nrows; row += (long long)NBLK * (ARB / 64))
   {
     const V* rows[4];
     const int mask = lattice_rows<V, P>(W, (int)row, L, rows);
@@ -269,7 +274,7 @@ __global__ void __launch_bounds__(RB)
       s[1] += (double)ri * (double)ri;
     }
   }
-  store_partials<2>(s, part);
+  store_partials<2, ARB>(s, part);
 }
 
 __global__ void __launch_bounds__(RB)
@@ -593,6 +598,12 @@ struct femtc_solver
   // dofmap): its lattice buffer in the inner precision
   bool inner_assembled = false;
   void* lattice = nullptr;
+  // the batch of check_every iterations as a hipGraph (captured on
+  // cap_stream at the first solve, replayed on `stream`)
+  bool use_graph = true;
+  bool graph_failed = false;
+  hipStream_t cap_stream = nullptr;
+  hipGraphExec_t graph = nullptr;
 };
 
 namespace femtc
@@ -654,6 +665,10 @@ void destroy(femtc_solver* S)
   dev_free(S->cg64);
   dev_free(S->epart);
   dev_free(S->lattice);
+  if (S->graph)
+    (void)hipGraphExecDestroy(S->graph);
+  if (S->cap_stream)
+    (void)hipStreamDestroy(S->cap_stream);
   delete S;
 }
 
@@ -725,9 +740,8 @@ LatticeGrid lattice_grid(const femtc_solver* S)
 // k_update on the atomically scattered Ap, or k_assemble_update on the
 // assembled scatter's cell values.
 template <typename V>
-int launch_update(femtc_solver* S)
+int launch_update(femtc_solver* S, hipStream_t s)
 {
-  hipStream_t s = S->stream;
   if (S->inner_assembled)
   {
     const V* W = (const V*)S->lattice;
@@ -739,7 +753,7 @@ int launch_update(femtc_solver* S)
     {
 #define FEMTC_ASM_UPDATE(PP)                                                  \
   case PP:                                                                    \
-    hipLaunchKernelGGL((k_assemble_update<V, PP>), dim3(NBLK), dim3(RB), 0,  \
+    hipLaunchKernelGGL((k_assemble_update<V, PP>), dim3(NBLK), dim3(ARB), 0, \
                        s, W, r, dinv, L, st, S->part);                        \
     break;
       FEMTC_ASM_UPDATE(1)
@@ -783,7 +797,7 @@ int time_step(femtc_solver* S, int reps, float* ms)
     if (st == FEMTC_OK)
       st = run(a);
     if (st == FEMTC_OK)
-      st = launch_update<V>(S);
+      st = launch_update<V>(S, S->stream);
   }
   if (st == FEMTC_OK && (hipEventRecord(e1, S->stream) != hipSuccess ||
                          hipEventSynchronize(e1) != hipSuccess ||
@@ -869,47 +883,100 @@ int sync_read(femtc_solver* S, void* host, const void* dev, size_t bytes)
                       hipGetErrorString(e_));                                 \
   } while (0)
 
+// check_every CG iterations in precision V, queued on stream s: operator
+// (with the p.Ap energies), alpha, update, beta, p-update.  Every kernel
+// returns at once after the state's `done` is set.
+template <typename V>
+int enqueue_batch(femtc_solver* S, hipStream_t s)
+{
+  ApplyReq a = inner_req(S, dtype_of<V>(), S->p, S->Ap);
+  a.skip = &S->st->done;
+  a.energy = S->epart; // p.Ap from the operator kernel itself
+  a.stream = s;
+  for (int k = 0; k < S->check_every; ++k)
+  {
+    int st = run(a);
+    if (st != FEMTC_OK)
+      return st;
+    hipLaunchKernelGGL(k_energy_partials, dim3(NBLK), dim3(RB), 0, s,
+                       (const double*)S->epart, S->nepart, S->part,
+                       (const PcgState*)S->st);
+    hipLaunchKernelGGL(k_alpha, dim3(1), dim3(RB), 0, s,
+                       (const double*)S->part, S->st);
+    if ((st = launch_update<V>(S, s)) != FEMTC_OK)
+      return st;
+    hipLaunchKernelGGL(k_beta, dim3(1), dim3(RB), 0, s,
+                       (const double*)S->part, S->st);
+    hipLaunchKernelGGL(k_pupdate<V>, dim3(NBLK), dim3(RB), 0, s, (V*)S->p,
+                       (V*)S->e, (const V*)S->r, (const V*)S->dinv, S->n,
+                       (const PcgState*)S->st);
+  }
+  return hip_status(hipGetLastError());
+}
+
+// One batch on the solver's stream: replayed from a hipGraph captured at
+// the first call (on a private stream of default priority; the graph holds
+// only kernel nodes), or queued kernel by kernel when graphs are off or
+// capture failed.
+template <typename V>
+int launch_batch(femtc_solver* S)
+{
+  if (S->use_graph && !S->graph && !S->graph_failed)
+  {
+    hipGraph_t g = nullptr;
+    bool ok = S->cap_stream ||
+              hipStreamCreateWithFlags(&S->cap_stream, hipStreamNonBlocking) ==
+                  hipSuccess;
+    if (ok)
+      ok = hipStreamBeginCapture(S->cap_stream,
+                                 hipStreamCaptureModeThreadLocal) == hipSuccess;
+    if (ok)
+    {
+      const int st = enqueue_batch<V>(S, S->cap_stream);
+      ok = hipStreamEndCapture(S->cap_stream, &g) == hipSuccess &&
+           st == FEMTC_OK && g &&
+           hipGraphInstantiate(&S->graph, g, nullptr, nullptr, 0) ==
+               hipSuccess;
+    }
+    if (g)
+      (void)hipGraphDestroy(g);
+    if (!ok)
+    {
+      S->graph = nullptr;
+      S->graph_failed = true;
+      (void)hipGetLastError();
+    }
+  }
+  if (S->graph)
+    return hipGraphLaunch(S->graph, S->stream) == hipSuccess
+               ? FEMTC_OK
+               : fail(FEMTC_ERR_HIP, std::string("solver graph launch: ") +
+                                         hipGetErrorString(hipGetLastError()));
+  const int st = enqueue_batch<V>(S, S->stream);
+  if (st == FEMTC_ERR_HIP)
+    return fail(st, std::string("solver kernel launch: ") +
+                        hipGetErrorString(hipGetLastError()));
+  return st;
+}
+
 // PCG in precision V on A e = rhs (inner buffers of S), from e = 0.
 template <typename V>
 int pcg_run(femtc_solver* S, double rtol, int maxit, int* iters, int* status)
 {
   hipStream_t s = S->stream;
   const int64_t n = S->n;
-  V* e = (V*)S->e;
-  V* r = (V*)S->r;
-  V* p = (V*)S->p;
-  V* Ap = (V*)S->Ap;
-  const V* dinv = (const V*)S->dinv;
   hipLaunchKernelGGL(k_start<V>, dim3(NBLK), dim3(RB), 0, s, (const V*)S->rhs,
-                     e, r, p, Ap, dinv, n, S->part);
+                     (V*)S->e, (V*)S->r, (V*)S->p, (V*)S->Ap,
+                     (const V*)S->dinv, n, S->part);
   hipLaunchKernelGGL(k_reduce_start, dim3(1), dim3(RB), 0, s,
                      (const double*)S->part, S->st, rtol, maxit);
   LAUNCH_CHECK();
-  ApplyReq a = inner_req(S, dtype_of<V>(), p, Ap);
-  a.skip = &S->st->done;
-  a.energy = S->epart; // p.Ap from the operator kernel itself
   for (;;)
   {
-    for (int k = 0; k < S->check_every; ++k)
-    {
-      int st = run(a);
-      if (st != FEMTC_OK)
-        return st;
-      hipLaunchKernelGGL(k_energy_partials, dim3(NBLK), dim3(RB), 0, s,
-                         (const double*)S->epart, S->nepart, S->part,
-                         (const PcgState*)S->st);
-      hipLaunchKernelGGL(k_alpha, dim3(1), dim3(RB), 0, s,
-                         (const double*)S->part, S->st);
-      if ((st = launch_update<V>(S)) != FEMTC_OK)
-        return fail(st, std::string("solver kernel launch: ") +
-                            hipGetErrorString(hipGetLastError()));
-      hipLaunchKernelGGL(k_beta, dim3(1), dim3(RB), 0, s,
-                         (const double*)S->part, S->st);
-      hipLaunchKernelGGL(k_pupdate<V>, dim3(NBLK), dim3(RB), 0, s, p, e,
-                         (const V*)r, dinv, n, (const PcgState*)S->st);
-      LAUNCH_CHECK();
-    }
-    int st = sync_read(S, S->hst, S->st, sizeof(PcgState));
+    int st = launch_batch<V>(S);
+    if (st != FEMTC_OK)
+      return st;
+    st = sync_read(S, S->hst, S->st, sizeof(PcgState));
     if (st != FEMTC_OK)
       return st;
     if (S->hst->done)
@@ -1219,6 +1286,28 @@ int femtc_solver_diagonal(const femtc_solver* solver, const double** diag)
   if (!solver || !diag)
     return fail(FEMTC_ERR_INVALID_ARGUMENT, "NULL argument");
   *diag = solver->diag;
+  return FEMTC_OK;
+}
+
+int femtc_solver_set_batch(femtc_solver* solver, int check_every,
+                           int use_graph)
+{
+  if (!solver)
+    return fail(FEMTC_ERR_INVALID_ARGUMENT, "solver is NULL");
+  if (check_every < 1 || check_every > 1024)
+    return fail(FEMTC_ERR_INVALID_ARGUMENT,
+                "check_every must be in [1, 1024]");
+  if (solver->graph &&
+      (check_every != solver->check_every || !use_graph))
+  {
+    if (hipStreamSynchronize(solver->stream) != hipSuccess)
+      return fail(FEMTC_ERR_HIP, "solver: stream synchronisation failed");
+    (void)hipGraphExecDestroy(solver->graph);
+    solver->graph = nullptr;
+  }
+  solver->check_every = check_every;
+  solver->use_graph = use_graph != 0;
+  solver->graph_failed = false;
   return FEMTC_OK;
 }
 

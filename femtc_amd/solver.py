@@ -54,6 +54,13 @@ class PCGSolver:
             "solver_create")
         self._h = h
 
+    def set_batch(self, check_every: int = 16, use_graph: bool = True) -> None:
+        """CG iterations queued between convergence checks, and whether the
+        batch is replayed from a captured hipGraph."""
+        _lib.check(_lib.lib().femtc_solver_set_batch(self._h, check_every,
+                                                     int(use_graph)),
+                   "solver_set_batch")
+
     def scatter(self) -> int:
         """Scatter of the CG iterations chosen at creation: SCATTER_DIRECT or
         SCATTER_ASSEMBLED."""

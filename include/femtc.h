@@ -395,6 +395,13 @@ int femtc_solver_diagonal(const femtc_solver* solver, const double** diag);
 /* The scatter the solver chose for its CG iterations
  * (FEMTC_SCATTER_DIRECT or FEMTC_SCATTER_ASSEMBLED). */
 int femtc_solver_scatter(const femtc_solver* solver, int* scatter);
+/* CG iterations queued between two reads of the convergence state
+ * (check_every, default 16; iterations past convergence cost a kernel
+ * launch each and do no work), and whether that batch is replayed from a
+ * hipGraph captured at the first solve (use_graph, default 1) or queued
+ * kernel by kernel. */
+int femtc_solver_set_batch(femtc_solver* solver, int check_every,
+                           int use_graph);
 int femtc_solver_destroy(femtc_solver* solver);
 
 /* ---------------------------------------------------------------------- */

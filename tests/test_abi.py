@@ -140,6 +140,9 @@ def test_solver_create_rejects_bad_problems_without_touching_the_device():
         _lib.ERR_INVALID_ARGUMENT
     assert "femtc_scatter" in _lib.last_error()
     assert h.value is None
+    assert L.femtc_solver_set_batch(None, 16, 1) == _lib.ERR_INVALID_ARGUMENT
+    s = C_.c_int32()
+    assert L.femtc_solver_scatter(None, C_.byref(s)) == _lib.ERR_INVALID_ARGUMENT
 
 
 def test_plan_create_validates_arguments_without_touching_the_device():

@@ -183,6 +183,41 @@ def test_solver_assembled_scatter(inner, op, nx, ny, nz, P, Q, affine):
     assert abs(iters[F.SCATTER_DIRECT] - iters[F.SCATTER_ASSEMBLED]) <= 2, iters
 
 
+@pytest.mark.parametrize("inner", [F.F64, F.F32])
+def test_solver_graph_replays_match_eager_launches(inner):
+    # with the assembled scatter every reduction of the CG iterations has a
+    # fixed order, so on one solver the batch replayed from the captured
+    # hipGraph, the same batch queued kernel by kernel, and another batch
+    # size give the same solution, solve after solve (eager host work and
+    # synchronisation in between): bit for bit in FP64; the refined solve's
+    # FP64 residual uses the atomic scatter, whose last bits vary, hence
+    # 1e-12 there
+    nx, ny, nz, P, Q = 3, 3, 4, 4, 5
+    C, G, Dw, dm, ng, tab, f = _problem(F.OP_KPM, nx, ny, nz, P, Q, 31)
+    args = (F.OP_KPM, dev(tab), dev(dm, torch.int32), C, P, Q, ng)
+    kw = dict(G=dev(G), detJw=dev(Dw), inner=inner, grid=(nx, ny, nz),
+              scatter=F.SCATTER_ASSEMBLED)
+    df = dev(f)
+    sols = []
+    with F.PCGSolver(*args, **kw) as S:
+        for check_every, graph in ((16, True), (16, False), (5, True),
+                                   (16, True)):
+            S.set_batch(check_every, graph)
+            for rep in range(2):
+                u = torch.empty(ng, device="cuda", dtype=torch.float64)
+                info = S.solve(df, u, rtol=1e-10)
+                assert info["relres"] <= 1.01e-10
+                sols.append(host(u))
+                _ = float(torch.rand(1, device="cuda").sum())  # eager work
+        with pytest.raises(F.FemtcError, match="check_every"):
+            S.set_batch(0, True)
+    for s in sols[1:]:
+        if inner == F.F64:
+            assert np.array_equal(s, sols[0])
+        else:
+            assert rel(s, sols[0]) <= 1e-12
+
+
 def test_solver_assembled_scatter_needs_the_structured_dofmap():
     nx, ny, nz, P, Q = 3, 2, 3, 2, 3
     C, G, Dw, dm, ng, tab, f = _problem(F.OP_KPM, nx, ny, nz, P, Q, 7)
