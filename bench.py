@@ -20,13 +20,15 @@ Beside the metric line's required fields it reports, on the same GPU:
              built -Ofast from the reference sources (oracle/_ref) on the host
              cores, on a bounded sample of cells of the same shape
   solve      (K+M)u = f on the same mesh to 1e-10: FP64 Jacobi-PCG vs FP32
-             inner PCG with FP64 defect correction (solves/s, error)
+             inner PCG with FP64 defect correction (solves/s, error), each
+             also with per-cell geometry and with the structured grid given
+             (assembled scatter)
   apply_p4_64cube  the P=4 64^3 stiffness apply (GDoF/s, roofline fraction)
   batch      200 uneven meshes in one batched, graph-captured launch set
   serving    a 500 req/s Poisson stream of applies and solves (p50/p99)
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--apply-only]
-                  [--no-cpu-baseline] [--copy-calibration MiB]
+                  [--only PART] [--no-cpu-baseline] [--copy-calibration MiB]
 """
 from __future__ import annotations
 
@@ -194,45 +196,77 @@ def time_apply(torch, F, pr, steps, warmup, x, y):
     return wall * 1e3 / steps, kern_ms
 
 
+def copy_bandwidth(torch, mib=2048, reps=10):
+    """Device-to-device copy rate (read + write bytes / time) of a 2 GiB
+    buffer: the practical HBM ceiling beside the 8 TB/s peak."""
+    m = mib * (1 << 20) // 8
+    src = torch.ones(m, device="cuda", dtype=torch.float64)
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    sec = e0.elapsed_time(e1) * 1e-3 / reps
+    del src, dst
+    torch.cuda.empty_cache()
+    return 2 * 8 * m / sec / 1e9
+
+
 def alg_bytes_stiffness(pr):
     return 8 * pr["ng"] * 2 + 4 * pr["C"] * pr["N"] ** 3 + 48 * pr["C"] * pr["Q"] ** 3
 
 
 def bench_solve(torch, F, n=48, P=6):
-    """(K+M)u = f to 1e-10: FP64 PCG vs FP32 inner + FP64 refinement."""
+    """(K+M)u = f to 1e-10: the plain FP64 PCG (the reference's data: per-
+    point geometry, dofmap, atomic scatter) against FP32 inner PCG + FP64
+    refinement, each also with per-cell geometry (affine cells) and with the
+    structured grid given (assembled scatter when it times faster)."""
     pr = structured(torch, F, n, P, want_detJw=True)
     f = torch.from_numpy(_uniform(7, pr["ng"], -1, 1)).cuda()
     out = {"workload": f"(K+M)u=f, {n}^3 mesh, P={P}, Q={P + 1}, f~U[-1,1], "
                        "rtol 1e-10 (FP64 residual)"}
     sols = {}
-    # stored per-point geometry (the reference's format), and the inner
-    # operator computing its geometry from the cell Jacobians (affine cells)
-    for name, inner, J in (("fp64_pcg", F.F64, None),
-                           ("fp32_inner_fp64_refine", F.F32, None),
-                           ("fp64_pcg_cell_geometry", F.F64, pr["J"]),
-                           ("fp32_inner_cell_geometry_fp64_refine", F.F32,
-                            pr["J"])):
+    grid = (n, n, n)
+    names = {F.SCATTER_DIRECT: "atomic", F.SCATTER_ASSEMBLED: "assembled"}
+    for name, inner, J, g in (
+            ("fp64_pcg", F.F64, None, None),
+            ("fp32_inner_fp64_refine", F.F32, None, None),
+            ("fp64_pcg_cell_geometry", F.F64, pr["J"], None),
+            ("fp32_inner_cell_geometry_fp64_refine", F.F32, pr["J"], None),
+            ("fp64_pcg_structured", F.F64, None, grid),
+            ("fp64_pcg_cell_geometry_structured", F.F64, pr["J"], grid),
+            ("fp32_inner_cell_geometry_structured_fp64_refine", F.F32,
+             pr["J"], grid)):
         with F.PCGSolver(F.OP_KPM, pr["tab"], pr["dm"], pr["C"], P, P + 1,
                          pr["ng"], G=pr["G"], detJw=pr["D"], inner=inner,
-                         J=J) as S:
+                         J=J, grid=g) as S:
             u = torch.empty(pr["ng"], device="cuda", dtype=torch.float64)
-            S.solve(f, u, rtol=1e-10)  # warm-up
+            S.solve(f, u, rtol=1e-10)  # warm-up (captures the batch graph)
             infos = [S.solve(f, u, rtol=1e-10) for _ in range(2)]
+            scatter = names[S.scatter()]
         ms = float(np.mean([i["seconds"] for i in infos])) * 1e3
         out[name] = {"ms_per_solve": ms, "solves_per_s": 1e3 / ms,
                      "iterations": infos[-1]["inner_iterations"],
                      "outer": infos[-1]["outer_iterations"],
-                     "relres": infos[-1]["relres"]}
+                     "relres": infos[-1]["relres"], "scatter": scatter}
         sols[name] = u
     u64 = sols["fp64_pcg"]
     base = out["fp64_pcg"]["ms_per_solve"]
+    best64 = min(v["ms_per_solve"] for k, v in out.items()
+                 if k.startswith("fp64_pcg"))
     for name, u in sols.items():
         if name != "fp64_pcg":
             out[name]["error_vs_fp64_solution"] = float((u - u64).norm() /
                                                         u64.norm())
             out[name]["speedup_vs_fp64_pcg"] = base / out[name]["ms_per_solve"]
-    out["speedup_refined_vs_fp64"] = out["fp32_inner_fp64_refine"][
-        "speedup_vs_fp64_pcg"]
+    best = "fp32_inner_cell_geometry_structured_fp64_refine"
+    out["speedup_refined_vs_fp64"] = out[best]["speedup_vs_fp64_pcg"]
+    out["speedup_refined_vs_fastest_fp64"] = best64 / out[best]["ms_per_solve"]
     out["bf16_mfma_inner"] = "not built"
     return out
 
@@ -243,11 +277,34 @@ def bench_apply_p4(torch, F, steps=30, warmup=60):
     y = torch.empty(pr["ng"], device="cuda", dtype=torch.float64)
     ms, kern = time_apply(torch, F, pr, steps, warmup, x, y)
     ab = alg_bytes_stiffness(pr)
+    # the same apply through an autotuned plan (direct, blocked or assembled
+    # scatter, whichever times fastest on these buffers)
+    plan = F.Plan(64, 64, 64, 4, pr["dm"], pr["ng"])
+    shape, _ = plan.autotune(F.OP_STIFFNESS, pr["tab"], x, y, 5, G=pr["G"],
+                             reps=5)
+    stream = torch.cuda.current_stream()
+    for _ in range(warmup):
+        plan.apply(F.OP_STIFFNESS, pr["tab"], x, y, 5, G=pr["G"])
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        plan.apply(F.OP_STIFFNESS, pr["tab"], x, y, 5, G=pr["G"])
+    e1.record(stream)
+    torch.cuda.synchronize()
+    pms = e0.elapsed_time(e1) / steps
+    plan.close()
     return {"workload": "stiffness apply y=Kx, 64^3 mesh, P=4 GLL, Q=5, "
                         "int32 dofmap, FP64",
             "gdofs": pr["ng"] / (ms * 1e-3) / 1e9, "kernel_ms": kern,
             "algorithmic_bytes": ab,
-            "roofline_frac": ab / (kern * 1e-3) / 1e9 / HBM_PEAK_GBS}
+            "roofline_frac": ab / (kern * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "plan": {"scatter": {(0, 0, 0): "atomic",
+                                 (-1, -1, -1): "assembled"}.get(
+                                     tuple(shape), f"blocked {tuple(shape)}"),
+                     "ms_per_apply": pms,
+                     "gdofs": pr["ng"] / (pms * 1e-3) / 1e9,
+                     "roofline_frac": ab / (pms * 1e-3) / 1e9 / HBM_PEAK_GBS}}
 
 
 def bench_batch(torch, F):
@@ -306,6 +363,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--apply-only", action="store_true",
                     help="skip the solve, P=4 apply, batch and serving parts")
+    ap.add_argument("--only", choices=("solve", "apply_p4_64cube", "batch",
+                                       "serving"),
+                    help="run the metric apply and this part only")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
                     help="only copy MiB three times (counter calibration)")
     a = ap.parse_args()
@@ -383,10 +443,18 @@ def main():
     if not a.apply_only:
         for key, fn in (("solve", bench_solve), ("apply_p4_64cube", bench_apply_p4),
                         ("batch", bench_batch), ("serving", bench_serving)):
+            if a.only and key != a.only:
+                continue
             print(f"[bench] {key}", file=sys.stderr, flush=True)
             out[key] = fn(torch, F)
             torch.cuda.empty_cache()
-        out["p99_solve_ms"] = out["serving"].get("solve_p99_ms")
+        if "serving" in out:
+            out["p99_solve_ms"] = out["serving"].get("solve_p99_ms")
+        if "solve" in out:
+            out["solve_speedup_refined_vs_fp64"] = out["solve"][
+                "speedup_refined_vs_fp64"]
+        if not a.only:
+            out["hbm_copy_GBs"] = copy_bandwidth(torch)
     if not a.no_cpu_baseline:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(P, C, ng)
