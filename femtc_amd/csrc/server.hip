@@ -14,6 +14,7 @@
 #include <thread>
 #include <vector>
 
+#include "assemble.hpp"
 #include "common.hpp"
 
 namespace femtc
@@ -33,6 +34,9 @@ size_t solver_bytes(const femtc_problem& p, int inner)
   b += 4 * vs * n;                                // rhs, e, p, A p
   if (p.J)
     b += 8 * CELLGEO * C;                         // per-cell geometry, FP64
+  if (p.grid[0] > 0)                              // assembled-scatter lattice
+    b += vs * (size_t)lattice_size(p.grid[0], p.grid[1], p.grid[2],
+                                   p.degree);
   if (inner != FEMTC_F64)
   {
     b += 2 * 4 * n;                               // r, D^-1 (FP32)

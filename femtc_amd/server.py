@@ -29,12 +29,14 @@ class ResidentProblem:
                                    _lib.SCATTER_AUTO)
 
     @classmethod
-    def from_item(cls, it, affine: bool = False):
+    def from_item(cls, it, affine: bool = False, structured: bool = False):
         """From a femtc_amd.workloads.structured_problem dict; affine=True
-        also hands the cell Jacobians to the solvers."""
+        also hands the cell Jacobians to the solvers, structured=True the
+        grid (the solvers may then use the assembled scatter)."""
         return cls(it["op"], it["degree"], it["nq"], it["tables"],
                    it["dofmap"], it["num_cells"], it["ndofs"], it.get("G"),
-                   it.get("detJw"), it.get("J") if affine else None)
+                   it.get("detJw"), it.get("J") if affine else None,
+                   it.get("dims") if structured else None)
 
 
 class Server:

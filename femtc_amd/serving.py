@@ -33,11 +33,14 @@ def make_pool(torch, seed=0, n_problems=16, n_lo=4, n_hi=12,
 
 def run_stream(torch, items, rate=500.0, duration=2.0, solve_frac=0.5,
                slots=4, hbm_budget=16 << 30, rtol=1e-10, seed=1,
-               inner=_lib.F32, max_requests=4000):
+               inner=_lib.F32, max_requests=4000, affine=False,
+               structured=False):
     """Feed the stream for `duration` seconds of arrivals, drain, and return
-    (stats, results)."""
+    (stats, results).  affine / structured: the resident problems also carry
+    their cell Jacobians / grid (ResidentProblem.from_item)."""
     rng = np.random.default_rng(seed)
-    probs = [ResidentProblem.from_item(it) for it in items]
+    probs = [ResidentProblem.from_item(it, affine=affine, structured=structured)
+             for it in items]
     # inputs and outputs prepared up front (device-resident)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, max_requests))
     arrivals = arrivals[arrivals < duration]
