@@ -95,24 +95,32 @@ __device__ inline T stream_load(const T* p)
 
 // Row d of a [rows][LEN] table, loaded at this point of the program: the
 // offset goes through an empty asm so the compiler cannot hoist the (scalar)
-// loads of every row to the top of the kernel.
+// loads of every row to the top of the kernel.  It is an unsigned byte
+// offset, which the scalar loads take as their SGPR offset (with the
+// element's immediate offset): no 64-bit address arithmetic per row.
 template <int LEN, typename T>
 __device__ inline void load_row(const T* __restrict__ tab, int off, T (&row)[LEN])
 {
-  asm volatile("" : "+s"(off));
+  unsigned boff = (unsigned)off * (unsigned)sizeof(T);
+  asm volatile("" : "+s"(boff));
+  const T* p =
+      reinterpret_cast<const T*>(reinterpret_cast<const char*>(tab) + boff);
 #pragma unroll
   for (int i = 0; i < LEN; ++i)
-    row[i] = tab[off + i];
+    row[i] = p[i];
 }
 
 // Column d of a table stored [Q][N] (reference mode backward, Q == N).
 template <int LEN, int STRIDE, typename T>
 __device__ inline void load_col(const T* __restrict__ tab, int off, T (&row)[LEN])
 {
-  asm volatile("" : "+s"(off));
+  unsigned boff = (unsigned)off * (unsigned)sizeof(T);
+  asm volatile("" : "+s"(boff));
+  const T* p =
+      reinterpret_cast<const T*>(reinterpret_cast<const char*>(tab) + boff);
 #pragma unroll
   for (int i = 0; i < LEN; ++i)
-    row[i] = tab[off + i * STRIDE];
+    row[i] = p[i * STRIDE];
 }
 
 template <typename T, int N, int Q, bool REF>
@@ -357,7 +365,9 @@ apply_body(const T* __restrict__ tab, const T* __restrict__ x,
       cell0 = blk * CPB;
       ncells_blk = num_cells - cell0 < CPB ? num_cells - cell0 : (long long)CPB;
     }
-    const long long cell = cell0 + lc;
+    // one cell per group (Q >= 6): the cell is wave-uniform, so its index
+    // arithmetic (64-bit products and divisions) runs on the scalar unit
+    const long long cell = CPB == 1 ? cell0 : cell0 + lc;
     const bool live = lane_ok && lc < ncells_blk;
 
     // ---------------- global loads issued up front ------------------------
