@@ -123,7 +123,7 @@ class BatchItem(C.Structure):
     _fields_ = [("op", _i32), ("dtype", _i32), ("degree", _i32), ("nq", _i32),
                 ("tables", _vp), ("x", _vp), ("y", _vp), ("G", _vp),
                 ("detJw", _vp), ("dofmap", _vp), ("num_cells", _i64),
-                ("ndofs", _i64)]
+                ("ndofs", _i64), ("grid", _i32 * 3)]
 
 
 REQUEST_APPLY, REQUEST_SOLVE = 0, 1

@@ -13,7 +13,9 @@ from .operators import _dtype_code, _ptr, _stream
 
 class Batch:
     """items: dicts with keys op, degree, nq, tables, x, y and, as needed,
-    G, detJw, dofmap (None: cell-local), num_cells, ndofs (continuous).
+    G, detJw, dofmap (None: cell-local), num_cells, ndofs (continuous), and
+    dims (nx, ny, nz) when the dofmap is the structured numbering of that
+    grid (checked; enables the assembled scatter for the item's group).
     Every tensor must stay alive and in place while the batch is used."""
 
     def __init__(self, items, stream=None):
@@ -21,11 +23,13 @@ class Batch:
         self._keep = []
         for i, it in enumerate(items):
             x = it["x"]
+            dims = it.get("dims") if it.get("dofmap") is not None else None
+            grid = (C.c_int32 * 3)(*(dims if dims is not None else (0, 0, 0)))
             arr[i] = _lib.BatchItem(
                 it["op"], _dtype_code(x), it["degree"], it["nq"],
                 _ptr(it["tables"]), _ptr(x), _ptr(it["y"]), _ptr(it.get("G")),
                 _ptr(it.get("detJw")), _ptr(it.get("dofmap")),
-                it["num_cells"], it.get("ndofs", 0))
+                it["num_cells"], it.get("ndofs", 0), grid)
             self._keep.append(it)
         h = C.c_void_p()
         _lib.check(_lib.lib().femtc_batch_create(C.byref(h), arr, len(items),
@@ -63,7 +67,8 @@ class Batch:
 
     def set_group(self, group: int, per_wg: int) -> None:
         """per_wg = 0: one launch per problem; k > 0: fused, k blocks per
-        workgroup."""
+        workgroup; -1: the assembled scatter per problem (structured
+        items only)."""
         _lib.check(_lib.lib().femtc_batch_set_group(self._h, group, per_wg),
                    "batch_set_group")
 

@@ -10,6 +10,7 @@
 #include <tuple>
 #include <vector>
 
+#include "assemble.hpp"
 #include "common.hpp"
 
 namespace femtc
@@ -58,17 +59,65 @@ int cells_per_wg(int nq) { return nq * nq <= 64 ? 64 / (nq * nq) : 1; }
 
 constexpr int NSTREAMS = 4;
 
+// Per problem: its verified structured grid (0 if none) and, once the
+// assembled scatter is selected for its group, its lattice buffer.
+struct Extra
+{
+  int grid[3] = {0, 0, 0};
+  void* lattice = nullptr;
+};
+
 struct Group
 {
   BatchReq req;
   size_t first = 0; // offset of its descriptors in the device array
   std::vector<BatchDesc> host;
+  std::vector<Extra> extra; // parallel to host
   long long cells = 0;
-  // 0: one fused apply_batch launch; 1: one launch per problem
+  bool structured = true; // every problem has a verified grid
+  // 0: one fused apply_batch launch; 1: one launch per problem; 2: the
+  // assembled scatter per problem
   int per_problem = 1;
 };
 
-// Launch a group: fused, or one direct launch per problem.
+// Lattice buffers of a group's problems (allocated once).
+int ensure_lattices(Group& g)
+{
+  if (!g.structured)
+    return fail(FEMTC_ERR_INVALID_ARGUMENT,
+                "assembled scatter: a problem of the group has no verified "
+                "structured grid");
+  const size_t es = g.req.dtype == FEMTC_F64 ? 8 : 4;
+  for (Extra& e : g.extra)
+  {
+    if (e.lattice)
+      continue;
+    const size_t bytes =
+        es * (size_t)lattice_size(e.grid[0], e.grid[1], e.grid[2],
+                                  g.req.N - 1);
+    if (hipMalloc(&e.lattice, bytes) != hipSuccess)
+    {
+      (void)hipGetLastError();
+      e.lattice = nullptr;
+      return fail(FEMTC_ERR_OUT_OF_MEMORY,
+                  "assembled scatter buffer allocation failed");
+    }
+  }
+  return FEMTC_OK;
+}
+
+void free_lattices(Group& g)
+{
+  for (Extra& e : g.extra)
+    if (e.lattice)
+    {
+      (void)hipFree(e.lattice);
+      e.lattice = nullptr;
+    }
+}
+
+// Launch a group: fused, or one launch per problem (atomic or assembled
+// scatter).
 int launch_group(const Group& g, hipStream_t s)
 {
   if (!g.per_problem)
@@ -78,9 +127,20 @@ int launch_group(const Group& g, hipStream_t s)
     BatchLauncher f = batch_launcher(r.N);
     return f ? f(r) : FEMTC_ERR_DEGREE_NOT_SUPPORTED;
   }
-  for (const BatchDesc& d : g.host)
+  for (size_t i = 0; i < g.host.size(); ++i)
   {
+    const BatchDesc& d = g.host[i];
     ApplyReq a;
+    if (g.per_problem == 2)
+    {
+      const Extra& e = g.extra[i];
+      a.assembled = 1;
+      a.lattice = e.lattice;
+      a.ncx = e.grid[0];
+      a.ncy = e.grid[1];
+      a.ncz = e.grid[2];
+      a.overwrite = 1; // y = A x, whatever y held
+    }
     a.op = g.req.op;
     a.dtype = g.req.dtype;
     a.N = g.req.N;
@@ -147,6 +207,8 @@ void release(femtc_batch* b)
     (void)hipFree(b->d_desc);
   if (b->d_fill)
     (void)hipFree(b->d_fill);
+  for (Group& g : b->groups)
+    free_lattices(g);
   delete b;
 }
 
@@ -173,6 +235,15 @@ int check_item(const femtc_batch_item& it, int i)
   if (it.dofmap && it.ndofs < 1)
     return fail(FEMTC_ERR_INVALID_ARGUMENT,
                 where + "a continuous item needs ndofs >= 1");
+  const bool nogrid = !it.grid[0] && !it.grid[1] && !it.grid[2];
+  if (!nogrid && (!it.dofmap || it.grid[0] < 1 || it.grid[1] < 1 ||
+                  it.grid[2] < 1 ||
+                  (int64_t)it.grid[0] * it.grid[1] * it.grid[2] !=
+                      it.num_cells))
+    return fail(FEMTC_ERR_INVALID_ARGUMENT,
+                where + "grid must be all zero, or positive with "
+                        "grid[0]*grid[1]*grid[2] == num_cells on a "
+                        "continuous item");
   return FEMTC_OK;
 }
 
@@ -261,7 +332,26 @@ int femtc_batch_create(femtc_batch** out, const femtc_batch_item* items,
     d.blk_begin = g.req.nblk_total;
     if (it.num_cells > 0)
     {
+      Extra e;
+      int ok = 0;
+      if (it.grid[0] > 0)
+      {
+        const int vs = verify_structured(it.dofmap, it.grid[0], it.grid[1],
+                                         it.grid[2], it.degree, it.ndofs,
+                                         (hipStream_t)stream, &ok);
+        if (vs != FEMTC_OK)
+        {
+          release(b);
+          return vs;
+        }
+      }
+      if (ok)
+        for (int k = 0; k < 3; ++k)
+          e.grid[k] = it.grid[k];
+      else
+        g.structured = false;
       g.host.push_back(d);
+      g.extra.push_back(e);
       g.req.nblk_total += (it.num_cells + cpw - 1) / cpw;
       g.cells += it.num_cells;
     }
@@ -400,11 +490,16 @@ int femtc_batch_autotune(femtc_batch* b, int reps)
   {
     double best = 1e300;
     int bestk = 1, best_pp = 1;
-    // k = 0: one launch per problem; k > 0: fused, k blocks per workgroup
-    for (int k : {0, 1, 2, 4, 8})
+    // k = 0: one launch per problem; k > 0: fused, k blocks per workgroup;
+    // k = -1: the assembled scatter per problem (structured groups, when
+    // its buffers fit)
+    const bool try_asm = g.structured && ensure_lattices(g) == FEMTC_OK;
+    for (int k : {0, 1, 2, 4, 8, -1})
     {
+      if (k < 0 && !try_asm)
+        continue;
       Group t = g;
-      t.per_problem = k == 0;
+      t.per_problem = k == 0 ? 1 : k < 0 ? 2 : 0;
       t.req.per_wg = k > 0 ? k : 1;
       st = launch_group(t, b->stream); // warm-up
       (void)hipEventRecord(e0, b->stream);
@@ -426,6 +521,8 @@ int femtc_batch_autotune(femtc_batch* b, int reps)
       break;
     g.req.per_wg = bestk;
     g.per_problem = best_pp;
+    if (g.per_problem != 2)
+      free_lattices(g);
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
@@ -479,16 +576,29 @@ int femtc_batch_group(const femtc_batch* b, int g, int* op, int* dtype,
   if (nitems)
     *nitems = r.nitems;
   if (per_wg)
-    *per_wg = b->groups[g].per_problem ? 0 : r.per_wg;
+  {
+    const int pp = b->groups[g].per_problem;
+    *per_wg = pp == 2 ? -1 : pp ? 0 : r.per_wg;
+  }
   return FEMTC_OK;
 }
 
 int femtc_batch_set_group(femtc_batch* b, int g, int per_wg)
 {
-  if (!b || g < 0 || g >= (int)b->groups.size() || per_wg < 0 || per_wg > 64)
+  if (!b || g < 0 || g >= (int)b->groups.size() || per_wg < -1 ||
+      per_wg > 64)
     return fail(FEMTC_ERR_INVALID_ARGUMENT, "no such group or bad per_wg");
-  b->groups[g].per_problem = per_wg == 0;
-  b->groups[g].req.per_wg = per_wg > 0 ? per_wg : 1;
+  Group& grp = b->groups[g];
+  if (per_wg < 0)
+  {
+    const int st = ensure_lattices(grp);
+    if (st != FEMTC_OK)
+      return st;
+  }
+  grp.per_problem = per_wg == 0 ? 1 : per_wg < 0 ? 2 : 0;
+  grp.req.per_wg = per_wg > 0 ? per_wg : 1;
+  if (grp.per_problem != 2)
+    free_lattices(grp);
   if (b->exec)
   {
     (void)hipGraphExecDestroy(b->exec);

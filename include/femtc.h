@@ -283,6 +283,10 @@ typedef struct femtc_batch_item
   const int32_t* dofmap; /* [num_cells][N^3], or NULL for cell-local */
   int64_t num_cells;
   int64_t ndofs; /* continuous items: length of x and y */
+  /* Optional (all zero when unknown): the nx x ny x nz grid when dofmap is
+   * the structured numbering (checked at batch creation); a group whose
+   * items all have one may run the assembled scatter. */
+  int grid[3];
 } femtc_batch_item;
 
 typedef struct femtc_batch femtc_batch; /* opaque */
@@ -293,14 +297,18 @@ int femtc_batch_run(femtc_batch* batch);
 /* The same through a hipGraph: captured at the first call (and after an
  * autotune), replayed afterwards. */
 int femtc_batch_graph_launch(femtc_batch* batch);
-/* Time, for every group, one launch per problem and the fused launch with
- * 1, 2, 4 and 8 blocks per workgroup (reps launches each) and keep the
- * fastest; synchronous; outputs are left undefined. */
+/* Time, for every group, one launch per problem, the fused launch with
+ * 1, 2, 4 and 8 blocks per workgroup and, when every item of the group has
+ * a verified grid, the assembled scatter per problem (reps launches each)
+ * and keep the fastest; synchronous; outputs are left undefined. */
 int femtc_batch_autotune(femtc_batch* batch, int reps);
 int femtc_batch_stats(const femtc_batch* batch, int* ngroups,
                       int64_t* total_cells, int64_t* total_dofs);
 /* per_wg = 0: the group runs one launch per problem; per_wg = k > 0: one
- * fused launch, k blocks per workgroup.  set_group drops a captured graph. */
+ * fused launch, k blocks per workgroup; per_wg = -1: the assembled scatter
+ * per problem (its lattice buffers are allocated then, C*(P+1)^3 values per
+ * item; FEMTC_ERR_INVALID_ARGUMENT unless every item has a verified grid).
+ * set_group drops a captured graph. */
 int femtc_batch_set_group(femtc_batch* batch, int group, int per_wg);
 int femtc_batch_group(const femtc_batch* batch, int group, int* op,
                       int* dtype, int* degree, int* nq, int* gather,

@@ -136,9 +136,58 @@ def test_batch_autotune_keeps_results():
     b.graph_launch()  # a graph captured before tuning is dropped by it
     b.autotune(reps=2)
     for g in b.groups():
-        assert g["per_wg"] in (0, 1, 2, 4, 8)
+        assert g["per_wg"] in (0, 1, 2, 4, 8) or (g["per_wg"] == -1 and
+                                                  g["gather"] == 1)
     b.graph_launch()
     torch.cuda.synchronize()
+    _check(items, 1e-12)
+
+
+@pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-12),
+                                       (torch.float32, 1e-5)])
+def test_batch_assembled_scatter_vs_alone_and_oracle(dtype, tol):
+    # continuous groups on the assembled scatter (every item carries its
+    # verified grid), cell-local groups as they are; eager, graph replay,
+    # and a graph replay after eager work
+    items = _items(dtype, seed0=800)
+    b = F.Batch(items)
+    for g, info in enumerate(b.groups()):
+        if info["gather"]:
+            b.set_group(g, -1)
+            assert b.groups()[g]["per_wg"] == -1
+        else:
+            with pytest.raises(F.FemtcError, match="structured grid"):
+                b.set_group(g, -1)
+    for it in items:
+        it["y"].fill_(7.0)
+    b.run()
+    _check(items, tol)
+    for rep in range(2):
+        for it in items:
+            it["y"].fill_(-3.0)
+            it["x"].mul_(0.5)
+        torch.cuda.synchronize()
+        b.graph_launch()
+        torch.cuda.synchronize()
+        _check(items, tol)
+
+
+def test_batch_assembled_scatter_needs_structured_items():
+    # a relabelled dofmap is not the structured numbering: its group keeps
+    # the atomic scatter, the others may switch
+    items = _items(torch.float64, seed0=900, specs=SPECS[:2])
+    it = items[1]
+    ng = it["ndofs"]
+    perm = torch.randperm(ng, generator=torch.Generator().manual_seed(1))
+    it["dofmap"] = perm.to(device="cuda", dtype=torch.int32)[it["dofmap"].long()]
+    b = F.Batch(items)
+    ops = {g["op"]: i for i, g in enumerate(b.groups())}
+    b.set_group(ops[items[0]["op"]], -1)
+    with pytest.raises(F.FemtcError, match="structured grid"):
+        b.set_group(ops[it["op"]], -1)
+    b.autotune(reps=1)
+    assert b.groups()[ops[it["op"]]]["per_wg"] != -1
+    b.run()
     _check(items, 1e-12)
 
 
