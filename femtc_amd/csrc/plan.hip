@@ -500,15 +500,17 @@ int femtc_plan_autotune(femtc_plan* plan, int op, int dtype, int nq,
   Shape bs{0, 0, 0};
   for (const Shape& c : cand)
   {
-    // warm-up, then time reps applies (accumulating into y)
-    st = launch_plan(plan, op, dtype, nq, tables, x, y, G, detJw, 1, c, s);
+    // warm-up, then time reps overwriting applies (each scatter with the
+    // clearing of y it needs: a fill kernel for the atomic and blocked
+    // ones, none for the assembled one)
+    st = launch_plan(plan, op, dtype, nq, tables, x, y, G, detJw, 0, c, s);
     if (st == FEMTC_ERR_INVALID_ARGUMENT)
       continue; // shape does not fit (LDS) for this op/dtype
     if (st != FEMTC_OK)
       break;
     (void)hipEventRecord(e0, s);
     for (int i = 0; i < reps && st == FEMTC_OK; ++i)
-      st = launch_plan(plan, op, dtype, nq, tables, x, y, G, detJw, 1, c, s);
+      st = launch_plan(plan, op, dtype, nq, tables, x, y, G, detJw, 0, c, s);
     (void)hipEventRecord(e1, s);
     if (st != FEMTC_OK || hipEventSynchronize(e1) != hipSuccess)
       break;

@@ -249,9 +249,10 @@ int femtc_plan_apply(const femtc_plan* plan, int op, int dtype, int nq,
                      const void* G, const void* detJw, int accumulate,
                      void* stream);
 /* Time every candidate block shape (and the direct and, for the structured
- * dofmap, the assembled scatter) on these buffers, reps applies each, and
- * keep the fastest for (nq, dtype, op).
- * y is left cleared.  best_us (may be NULL) receives its time per apply. */
+ * dofmap, the assembled scatter) on these buffers, reps overwriting applies
+ * each (y = A x, clearing included), and keep the fastest for
+ * (nq, dtype, op).  y is left cleared.  best_us (may be NULL) receives its
+ * time per apply. */
 int femtc_plan_autotune(femtc_plan* plan, int op, int dtype, int nq,
                         const void* tables, const void* x, void* y,
                         const void* G, const void* detJw, int reps,
