@@ -249,11 +249,13 @@ def bench_solve(torch, F, n=48, P=6):
             S.solve(f, u, rtol=1e-10)  # warm-up (captures the batch graph)
             infos = [S.solve(f, u, rtol=1e-10) for _ in range(2)]
             scatter = names[S.scatter()]
+            even_odd = S.even_odd()
         ms = float(np.mean([i["seconds"] for i in infos])) * 1e3
         out[name] = {"ms_per_solve": ms, "solves_per_s": 1e3 / ms,
                      "iterations": infos[-1]["inner_iterations"],
                      "outer": infos[-1]["outer_iterations"],
-                     "relres": infos[-1]["relres"], "scatter": scatter}
+                     "relres": infos[-1]["relres"], "scatter": scatter,
+                     "even_odd": even_odd}
         sols[name] = u
     u64 = sols["fp64_pcg"]
     base = out["fp64_pcg"]["ms_per_solve"]

@@ -80,6 +80,8 @@ SIGNATURES = {
                                 _i32, _vp]),
     "femtc_plan_autotune": (_i32, [_vp, _i32, _i32, _i32, _vp, _vp, _vp, _vp,
                                    _vp, _i32, _vp, C.POINTER(C.c_double)]),
+    "femtc_plan_set_tables": (_i32, [_vp, _i32, _i32, _vp, _vp,
+                                     C.POINTER(_i32)]),
     "femtc_batch_create": (_i32, [C.POINTER(_vp), _vp, _i32, _vp]),
     "femtc_batch_run": (_i32, [_vp]),
     "femtc_batch_graph_launch": (_i32, [_vp]),
@@ -102,6 +104,7 @@ SIGNATURES = {
                                   _vp]),
     "femtc_solver_diagonal": (_i32, [_vp, C.POINTER(_vp)]),
     "femtc_solver_scatter": (_i32, [_vp, C.POINTER(_i32)]),
+    "femtc_solver_even_odd": (_i32, [_vp, C.POINTER(_i32)]),
     "femtc_solver_set_batch": (_i32, [_vp, _i32, _i32]),
     "femtc_solver_destroy": (_i32, [_vp]),
 }
@@ -112,7 +115,7 @@ class Problem(C.Structure):
     _fields_ = [("op", _i32), ("degree", _i32), ("nq", _i32),
                 ("num_cells", _i64), ("ndofs", _i64), ("dofmap", _vp),
                 ("tables", _vp), ("G", _vp), ("detJw", _vp), ("J", _vp),
-                ("grid", _i32 * 3), ("scatter", _i32)]
+                ("grid", _i32 * 3), ("scatter", _i32), ("even_odd", _i32)]
 
 
 SCATTER_AUTO, SCATTER_DIRECT, SCATTER_ASSEMBLED = 0, 1, 2

@@ -53,11 +53,14 @@
 // (forward: acc[q] += T[d][q] in[d]; backward: out[d] = sum_q T[d][q] in[q]),
 // and each row is fetched with scalar loads pinned inside its iteration, so
 // only a row or two of each table is live in SGPRs instead of the whole
-// 2 N Q table (which does not fit and would spill).
+// 2 N Q table (which does not fit and would spill).  The EO instances take
+// the even-odd table set of even_odd.hpp instead and contract each line as
+// two half-size products.
 #pragma once
 
 #include "assemble.hpp"
 #include "common.hpp"
+#include "even_odd.hpp"
 #include "lds_layout.hpp"
 
 namespace femtc
@@ -130,6 +133,100 @@ __device__ inline void load_bwd_row(const T* __restrict__ tab, int d, T (&row)[Q
     load_col<Q, N>(tab, d, row);
   else
     load_row<Q>(tab, d * Q, row);
+}
+
+// ---- even-odd contractions (table set of even_odd.hpp) --------------------
+
+// A padded even-odd table row (LEN a multiple of 4, 16-element aligned
+// offset) as LEN/4 four-element loads: one scalar load of 4 (FP32) or 8
+// (FP64) dwords each.
+template <int LEN, typename T>
+__device__ inline void load_row4(const T* __restrict__ tab, int off, T (&row)[LEN])
+{
+  static_assert(LEN % 4 == 0, "padded row");
+  typedef T V4 __attribute__((ext_vector_type(4)));
+  unsigned boff = (unsigned)off * (unsigned)sizeof(T);
+  asm volatile("" : "+s"(boff));
+  const V4* p =
+      reinterpret_cast<const V4*>(reinterpret_cast<const char*>(tab) + boff);
+#pragma unroll
+  for (int i = 0; i < LEN / 4; ++i)
+  {
+    const V4 v = p[i];
+    row[4 * i] = v.x;
+    row[4 * i + 1] = v.y;
+    row[4 * i + 2] = v.z;
+    row[4 * i + 3] = v.w;
+  }
+}
+
+// e[k] = v[k] + v[K-1-k], o[k] = v[k] - v[K-1-k], e[K/2] = v[K/2] (odd K)
+template <int K, typename T>
+__device__ inline void eo_split(const T (&v)[K], T (&e)[(K + 1) / 2],
+                                T (&o)[K / 2])
+{
+#pragma unroll
+  for (int k = 0; k < K / 2; ++k)
+  {
+    e[k] = v[k] + v[K - 1 - k];
+    o[k] = v[k] - v[K - 1 - k];
+  }
+  if constexpr (K % 2)
+    e[K / 2] = v[K / 2];
+}
+
+// One table term (term dims of eo_dims(K, M, S), at element offset off):
+// S = +1: ev += A e, od += B o;  S = -1: ev += A o, od += B e.
+// INIT: the first term into ev/od (products instead of sums).
+template <int K, int M, int S, bool INIT, typename T>
+__device__ inline void eo_term(const T* __restrict__ tab, int off,
+                               const T (&e)[(K + 1) / 2], const T (&o)[K / 2],
+                               T (&ev)[(M + 1) / 2], T (&od)[M / 2])
+{
+  constexpr EoDims d = eo_dims(K, M, S);
+#pragma unroll
+  for (int k = 0; k < d.rowsA; ++k)
+  {
+    T row[d.RA];
+    load_row4<d.RA>(tab, off + k * d.RA, row);
+    T in;
+    if constexpr (S > 0)
+      in = e[k];
+    else
+      in = o[k];
+#pragma unroll
+    for (int m = 0; m < d.Mp; ++m)
+      ev[m] = (INIT && k == 0) ? row[m] * in : fma(row[m], in, ev[m]);
+  }
+#pragma unroll
+  for (int k = 0; k < d.rowsB; ++k)
+  {
+    T row[d.RB];
+    load_row4<d.RB>(tab, off + d.rowsA * d.RA + k * d.RB, row);
+    T in;
+    if constexpr (S > 0)
+      in = o[k];
+    else
+      in = e[k];
+#pragma unroll
+    for (int m = 0; m < d.Hm; ++m)
+      od[m] = (INIT && k == 0) ? row[m] * in : fma(row[m], in, od[m]);
+  }
+}
+
+// out[m] = ev[m] + od[m], out[M-1-m] = ev[m] - od[m], out[M/2] = ev[M/2]
+template <int M, typename T>
+__device__ inline void eo_join(const T (&ev)[(M + 1) / 2], const T (&od)[M / 2],
+                               T (&out)[M])
+{
+#pragma unroll
+  for (int m = 0; m < M / 2; ++m)
+  {
+    out[m] = ev[m] + od[m];
+    out[M - 1 - m] = ev[m] - od[m];
+  }
+  if constexpr (M % 2)
+    out[M / 2] = ev[M / 2];
 }
 
 // The geometry block is staged through LDS in two halves of qa-planes:
@@ -229,7 +326,9 @@ inline size_t blocked_box_bytes(size_t box)
 // cell-dof value is stored (no atomics) into the lattice-ordered buffer y of
 // lattice_index(), and a second pass (assemble.hpp) sums the values of each
 // global dof.  Cells are numbered (ci*ncy + cj)*ncz + ck.
-template <typename T, int N, int Q, int OP, bool REF, bool GATHER, int BW>
+// EO: tab is the even-odd table set (even_odd.hpp) of symmetric tables.
+template <typename T, int N, int Q, int OP, bool REF, bool GATHER, int BW,
+          bool EO = false>
 __device__ __forceinline__ void
 apply_body(const T* __restrict__ tab, const T* __restrict__ x,
            T* __restrict__ y, const T* __restrict__ G,
@@ -252,7 +351,13 @@ apply_body(const T* __restrict__ tab, const T* __restrict__ x,
   constexpr bool FULL = (OPB == OP_STIFF || OPB == OP_KPM); // 3 gradients
   constexpr bool GEO = GRAD && !AFF; // geometry streamed per point
   static_assert(!AFF || (!LIT && !REF && !BLK), "affine geometry variants");
+  static_assert(!EO || (!LIT && !REF && !BLK), "even-odd variants");
   constexpr bool UQ = MASSTERM || LIT;
+  // even-odd table set: term offsets, the weights, half lengths
+  constexpr int EPF = eo_offset(N, Q, EO_PHI_F), EGF = eo_offset(N, Q, EO_DPHI_F);
+  constexpr int EPB = eo_offset(N, Q, EO_PHI_B), EGB = eo_offset(N, Q, EO_DPHI_B);
+  constexpr int WOFF = EO ? eo_offset(N, Q, EO_WEIGHTS) : 2 * N * Q;
+  constexpr int NP = (N + 1) / 2, NH = N / 2, QP = (Q + 1) / 2, QH = Q / 2;
   constexpr int NX = GRAD ? 2 : 1;
   constexpr int NY = GRAD ? 3 : 1;
   // X[qb][c][a] and Y[a][qb][qc] with bank-conflict-minimising strides
@@ -421,7 +526,7 @@ apply_body(const T* __restrict__ tab, const T* __restrict__ x,
 #pragma unroll
         for (int k = 0; k < CELLGEO; ++k)
           cg[k] = G[cell * CELLGEO + k];
-        const T* wq = tab + 2 * N * Q;
+        const T* wq = tab + WOFF;
         wbc = wq[ln / Q] * wq[ln - (ln / Q) * Q];
       }
     }
@@ -457,19 +562,34 @@ apply_body(const T* __restrict__ tab, const T* __restrict__ x,
     if (live && role1)
     {
       T ap[Q], ag[Q];
-#pragma unroll
-      for (int b = 0; b < N; ++b)
+      if constexpr (EO)
       {
-        T rp[Q], rg[Q];
-        load_row<Q>(ph, b * Q, rp);
+        T ue[NP], uo[NH], ev[QP], od[QH];
+        eo_split<N>(u, ue, uo);
+        eo_term<N, Q, 1, true>(tab, EPF, ue, uo, ev, od);
+        eo_join<Q>(ev, od, ap);
         if constexpr (GRAD)
-          load_row<Q>(gr, b * Q, rg);
-#pragma unroll
-        for (int q = 0; q < Q; ++q)
         {
-          ap[q] = b == 0 ? rp[q] * u[b] : fma(rp[q], u[b], ap[q]);
+          eo_term<N, Q, -1, true>(tab, EGF, ue, uo, ev, od);
+          eo_join<Q>(ev, od, ag);
+        }
+      }
+      else
+      {
+#pragma unroll
+        for (int b = 0; b < N; ++b)
+        {
+          T rp[Q], rg[Q];
+          load_row<Q>(ph, b * Q, rp);
           if constexpr (GRAD)
-            ag[q] = b == 0 ? rg[q] * u[b] : fma(rg[q], u[b], ag[q]);
+            load_row<Q>(gr, b * Q, rg);
+#pragma unroll
+          for (int q = 0; q < Q; ++q)
+          {
+            ap[q] = b == 0 ? rp[q] * u[b] : fma(rp[q], u[b], ap[q]);
+            if constexpr (GRAD)
+              ag[q] = b == 0 ? rg[q] * u[b] : fma(rg[q], u[b], ag[q]);
+          }
         }
       }
 #pragma unroll
@@ -494,21 +614,40 @@ apply_body(const T* __restrict__ tab, const T* __restrict__ x,
         if constexpr (GRAD)
           ing[c] = Xc[X_G * SX + r2q * XQ + c * XC + r2a];
       }
-#pragma unroll
-      for (int c = 0; c < N; ++c)
+      if constexpr (EO)
       {
-        T rp[Q], rg[Q];
-        load_row<Q>(ph, c * Q, rp);
+        T pe[NP], po[NH], ev[QP], od[QH];
+        eo_split<N>(inp, pe, po);
+        eo_term<N, Q, 1, true>(tab, EPF, pe, po, ev, od);
+        eo_join<Q>(ev, od, bpp);
         if constexpr (GRAD)
-          load_row<Q>(gr, c * Q, rg);
-#pragma unroll
-        for (int q = 0; q < Q; ++q)
         {
-          bpp[q] = c == 0 ? rp[q] * inp[c] : fma(rp[q], inp[c], bpp[q]);
+          eo_term<N, Q, -1, true>(tab, EGF, pe, po, ev, od);
+          eo_join<Q>(ev, od, bpg);
+          T ge[NP], go[NH];
+          eo_split<N>(ing, ge, go);
+          eo_term<N, Q, 1, true>(tab, EPF, ge, go, ev, od);
+          eo_join<Q>(ev, od, bgp);
+        }
+      }
+      else
+      {
+#pragma unroll
+        for (int c = 0; c < N; ++c)
+        {
+          T rp[Q], rg[Q];
+          load_row<Q>(ph, c * Q, rp);
           if constexpr (GRAD)
+            load_row<Q>(gr, c * Q, rg);
+#pragma unroll
+          for (int q = 0; q < Q; ++q)
           {
-            bpg[q] = c == 0 ? rg[q] * inp[c] : fma(rg[q], inp[c], bpg[q]);
-            bgp[q] = c == 0 ? rp[q] * ing[c] : fma(rp[q], ing[c], bgp[q]);
+            bpp[q] = c == 0 ? rp[q] * inp[c] : fma(rp[q], inp[c], bpp[q]);
+            if constexpr (GRAD)
+            {
+              bpg[q] = c == 0 ? rg[q] * inp[c] : fma(rg[q], inp[c], bpg[q]);
+              bgp[q] = c == 0 ? rp[q] * ing[c] : fma(rp[q], ing[c], bgp[q]);
+            }
           }
         }
       }
@@ -561,7 +700,33 @@ apply_body(const T* __restrict__ tab, const T* __restrict__ x,
 
     // F3: contract axis a
     T uq[Q], da[Q], db[Q], dc[Q];
-    if (live)
+    if (live && EO)
+    {
+      T pe[NP], po[NH], ev[QP], od[QH];
+      eo_split<N>(ipp, pe, po);
+      if constexpr (UQ)
+      {
+        eo_term<N, Q, 1, true>(tab, EPF, pe, po, ev, od);
+        eo_join<Q>(ev, od, uq);
+      }
+      if constexpr (GRAD)
+      {
+        eo_term<N, Q, -1, true>(tab, EGF, pe, po, ev, od);
+        eo_join<Q>(ev, od, da);
+        T ge[NP], go[NH];
+        eo_split<N>(igp, ge, go);
+        eo_term<N, Q, 1, true>(tab, EPF, ge, go, ev, od);
+        eo_join<Q>(ev, od, db);
+      }
+      if constexpr (FULL)
+      {
+        T ce[NP], co[NH];
+        eo_split<N>(ipg, ce, co);
+        eo_term<N, Q, 1, true>(tab, EPF, ce, co, ev, od);
+        eo_join<Q>(ev, od, dc);
+      }
+    }
+    else if (live)
     {
 #pragma unroll
       for (int a = 0; a < N; ++a)
@@ -607,7 +772,7 @@ apply_body(const T* __restrict__ tab, const T* __restrict__ x,
       if constexpr (AFF)
       {
         // G(q) = w(qa) w(qb) w(qc) S, detJw(q) = w(qa) w(qb) w(qc) detJ
-        const T wq = tab[2 * N * Q + q] * wbc;
+        const T wq = tab[WOFF + q] * wbc;
         if constexpr (GRAD)
         {
 #pragma unroll
@@ -662,31 +827,65 @@ apply_body(const T* __restrict__ tab, const T* __restrict__ x,
 #pragma unroll
       for (int q = QA1; q < Q; ++q)
         pointwise(q, QA1, 3 * (Q - QA1) * Q2);
-      // B3: contract axis a (transposed), one output row a at a time
-#pragma unroll
-      for (int a = 0; a < N; ++a)
+      // B3: contract axis a (transposed)
+      if constexpr (EO)
       {
-        T rp[Q], rg[Q];
-        load_bwd_row<T, N, Q, REF>(ph, a, rp);
+        T ev[NP], od[NH];
         if constexpr (FULL)
-          load_bwd_row<T, N, Q, REF>(gr, a, rg);
-        T ra = 0, rb = 0, rc = 0;
-#pragma unroll
-        for (int q = 0; q < Q; ++q)
         {
-          if constexpr (FULL)
-          {
-            ra = fma(rg[q], va[q], ra);
-            rb = fma(rp[q], vb[q], rb);
-          }
-          if constexpr (GRAD)
-            rc = fma(rp[q], vc[q], rc);
-          if constexpr (MASSTERM)
-            ra = fma(rp[q], vm[q], ra);
+          T xe[QP], xo[QH];
+          eo_split<Q>(va, xe, xo);
+          eo_term<Q, N, -1, true>(tab, EGB, xe, xo, ev, od);
         }
-        ra_[a] = ra;
-        rb_[a] = rb;
-        rc_[a] = rc;
+        if constexpr (MASSTERM)
+        {
+          T me[QP], mo[QH];
+          eo_split<Q>(vm, me, mo);
+          eo_term<Q, N, 1, !FULL>(tab, EPB, me, mo, ev, od);
+        }
+        eo_join<N>(ev, od, ra_);
+        if constexpr (FULL)
+        {
+          T be[QP], bo[QH];
+          eo_split<Q>(vb, be, bo);
+          eo_term<Q, N, 1, true>(tab, EPB, be, bo, ev, od);
+          eo_join<N>(ev, od, rb_);
+        }
+        if constexpr (GRAD)
+        {
+          T ce[QP], co[QH];
+          eo_split<Q>(vc, ce, co);
+          eo_term<Q, N, 1, true>(tab, EPB, ce, co, ev, od);
+          eo_join<N>(ev, od, rc_);
+        }
+      }
+      else
+      {
+#pragma unroll
+        for (int a = 0; a < N; ++a)
+        {
+          T rp[Q], rg[Q];
+          load_bwd_row<T, N, Q, REF>(ph, a, rp);
+          if constexpr (FULL)
+            load_bwd_row<T, N, Q, REF>(gr, a, rg);
+          T ra = 0, rb = 0, rc = 0;
+#pragma unroll
+          for (int q = 0; q < Q; ++q)
+          {
+            if constexpr (FULL)
+            {
+              ra = fma(rg[q], va[q], ra);
+              rb = fma(rp[q], vb[q], rb);
+            }
+            if constexpr (GRAD)
+              rc = fma(rp[q], vc[q], rc);
+            if constexpr (MASSTERM)
+              ra = fma(rp[q], vm[q], ra);
+          }
+          ra_[a] = ra;
+          rb_[a] = rb;
+          rc_[a] = rc;
+        }
       }
     }
     if constexpr (GEO)
@@ -736,7 +935,38 @@ apply_body(const T* __restrict__ tab, const T* __restrict__ x,
       }
     }
     stage_sync<BLK>(); // Y is read; X overwrites the same LDS
-    if (live && role2)
+    if (live && role2 && EO)
+    {
+      T s0[N], sb[N], ev[NP], od[NH];
+      {
+        T ae[QP], ao[QH];
+        eo_split<Q>(ra, ae, ao);
+        eo_term<Q, N, 1, true>(tab, EPB, ae, ao, ev, od);
+      }
+      if constexpr (GRAD)
+      {
+        T ce[QP], co[QH];
+        eo_split<Q>(rc, ce, co);
+        eo_term<Q, N, -1, false>(tab, EGB, ce, co, ev, od);
+      }
+      eo_join<N>(ev, od, s0);
+      if constexpr (FULL)
+      {
+        T be[QP], bo[QH];
+        eo_split<Q>(rb, be, bo);
+        eo_term<Q, N, 1, true>(tab, EPB, be, bo, ev, od);
+        eo_join<N>(ev, od, sb);
+      }
+#pragma unroll
+      for (int c = 0; c < N; ++c)
+      {
+        const int xo = r2q * XQ + c * XC + r2a;
+        Xc[X_S0 * SX + xo] = s0[c];
+        if constexpr (FULL)
+          Xc[X_SB * SX + xo] = sb[c];
+      }
+    }
+    else if (live && role2)
     {
 #pragma unroll
       for (int c = 0; c < N; ++c)
@@ -785,20 +1015,43 @@ apply_body(const T* __restrict__ tab, const T* __restrict__ x,
         wrow = (long long)ba.ncz * N;
         wpos = ((ci * N + r1a) * ba.ncy + cj) * N * wrow + ck * N + r1c;
       }
+      // even-odd: every output of the line at once
+      T wv[EO ? N : 1];
+      if constexpr (EO)
+      {
+        T ev[NP], od[NH];
+        {
+          T se[QP], so[QH];
+          eo_split<Q>(s0, se, so);
+          eo_term<Q, N, 1, true>(tab, EPB, se, so, ev, od);
+        }
+        if constexpr (FULL)
+        {
+          T be[QP], bo[QH];
+          eo_split<Q>(sb, be, bo);
+          eo_term<Q, N, -1, false>(tab, EGB, be, bo, ev, od);
+        }
+        eo_join<N>(ev, od, wv);
+      }
 #pragma unroll
       for (int b = 0; b < N; ++b)
       {
-        T rp[Q], rg[Q];
-        load_bwd_row<T, N, Q, REF>(ph, b, rp);
-        if constexpr (FULL)
-          load_bwd_row<T, N, Q, REF>(gr, b, rg);
         T w = 0;
-#pragma unroll
-        for (int q = 0; q < Q; ++q)
+        if constexpr (EO)
+          w = wv[b];
+        else
         {
-          w = fma(rp[q], s0[q], w);
+          T rp[Q], rg[Q];
+          load_bwd_row<T, N, Q, REF>(ph, b, rp);
           if constexpr (FULL)
-            w = fma(rg[q], sb[q], w);
+            load_bwd_row<T, N, Q, REF>(gr, b, rg);
+#pragma unroll
+          for (int q = 0; q < Q; ++q)
+          {
+            w = fma(rp[q], s0[q], w);
+            if constexpr (FULL)
+              w = fma(rg[q], sb[q], w);
+          }
         }
         if constexpr (BLK)
         {
@@ -864,7 +1117,8 @@ constexpr int min_waves_per_simd()
              : 1;
 }
 
-template <typename T, int N, int Q, int OP, bool REF, bool GATHER, int BW>
+template <typename T, int N, int Q, int OP, bool REF, bool GATHER, int BW,
+          bool EO = false>
 __global__ void __launch_bounds__(BW > 0 ? 64 * BW : block_threads<Q>(),
                                   (min_waves_per_simd<T, Q, OP, BW>()))
     apply_cells(const T* __restrict__ tab, const T* __restrict__ x,
@@ -887,8 +1141,8 @@ __global__ void __launch_bounds__(BW > 0 ? 64 * BW : block_threads<Q>(),
     if (blk >= ba.nblk)
       return; // uniform over the workgroup
   }
-  apply_body<T, N, Q, OP, REF, GATHER, BW>(tab, x, y, G, detJw, dofmap,
-                                           num_cells, ba, blk, epart);
+  apply_body<T, N, Q, OP, REF, GATHER, BW, EO>(tab, x, y, G, detJw, dofmap,
+                                               num_cells, ba, blk, epart);
 }
 
 // Batched launch: many independent problems of one (T, N, Q, OP, GATHER)
@@ -991,7 +1245,8 @@ int launch_batch_n(const BatchReq& r)
 inline int cells_per_workgroup(int nq) { return nq * nq <= 64 ? 64 / (nq * nq) : 1; }
 
 // Launch one instance: one workgroup per batch of CPB cells.
-template <typename T, int N, int Q, int OP, bool REF, bool GATHER>
+template <typename T, int N, int Q, int OP, bool REF, bool GATHER,
+          bool EO = false>
 int launch_cells(const ApplyReq& r)
 {
   constexpr int CPB = cells_per_block<Q>();
@@ -1000,9 +1255,10 @@ int launch_cells(const ApplyReq& r)
     return FEMTC_OK;
   if (nblk > 0x7fffffffLL)
     return FEMTC_ERR_INVALID_ARGUMENT;
-  hipLaunchKernelGGL((apply_cells<T, N, Q, OP, REF, GATHER, 0>),
+  hipLaunchKernelGGL((apply_cells<T, N, Q, OP, REF, GATHER, 0, EO>),
                      dim3((unsigned)nblk), dim3(block_threads<Q>()), 0,
-                     r.stream, (const T*)r.tab, (const T*)r.x, (T*)r.y,
+                     r.stream, (const T*)(EO ? r.eo_tab : r.tab),
+                     (const T*)r.x, (T*)r.y,
                      (const T*)r.G, (const T*)r.detJw, r.dofmap,
                      (long long)r.num_cells, r.skip, BlockArgs{},
                      r.energy);
@@ -1066,7 +1322,7 @@ int launch_blocked(const ApplyReq& r)
 
 // Assembled scatter: the cell pass into r.lattice, then (assembled == 1) the
 // assembly pass into y, stream-ordered.
-template <typename T, int N, int Q, int OP>
+template <typename T, int N, int Q, int OP, bool EO = false>
 int launch_assembled(const ApplyReq& r)
 {
   constexpr int CPB = cells_per_block<Q>();
@@ -1082,9 +1338,11 @@ int launch_assembled(const ApplyReq& r)
   ba.ncz = r.ncz;
   ba.nblk = (int)nblk;
   const long long grid = 8 * ((nblk + 7) / 8);
-  hipLaunchKernelGGL((apply_cells<T, N, Q, OP, false, true, SCATTER_ASSEMBLED>),
+  hipLaunchKernelGGL((apply_cells<T, N, Q, OP, false, true, SCATTER_ASSEMBLED,
+                                  EO>),
                      dim3((unsigned)grid), dim3(block_threads<Q>()), 0,
-                     r.stream, (const T*)r.tab, (const T*)r.x, (T*)r.lattice,
+                     r.stream, (const T*)(EO ? r.eo_tab : r.tab),
+                     (const T*)r.x, (T*)r.lattice,
                      (const T*)r.G, (const T*)r.detJw, r.dofmap,
                      (long long)r.num_cells, r.skip, ba, r.energy);
   if (r.assembled == 1)
@@ -1099,10 +1357,36 @@ int launch_assembled(const ApplyReq& r)
   return hip_status(hipGetLastError());
 }
 
+// Even-odd instances (Q == N only): direct or assembled scatter.
+template <typename T, int N, int Q, int OP>
+int launch_eo(const ApplyReq& r)
+{
+  return r.assembled ? launch_assembled<T, N, Q, OP, true>(r)
+                     : launch_cells<T, N, Q, OP, false, true, true>(r);
+}
+
 template <typename T, int N, int Q>
 int dispatch_op(const ApplyReq& r)
 {
   const bool g = r.gather != 0;
+  if (r.eo_tab)
+  {
+    if constexpr (N == Q)
+    {
+      if (!g || r.reference_mode || r.blocked)
+        return FEMTC_ERR_INVALID_ARGUMENT;
+      switch (r.op)
+      {
+      case OP_MASS: return launch_eo<T, N, Q, OP_MASS>(r);
+      case OP_STIFF: return launch_eo<T, N, Q, OP_STIFF>(r);
+      case OP_KPM: return launch_eo<T, N, Q, OP_KPM>(r);
+      case OP_MASS_AFF: return launch_eo<T, N, Q, OP_MASS_AFF>(r);
+      case OP_STIFF_AFF: return launch_eo<T, N, Q, OP_STIFF_AFF>(r);
+      case OP_KPM_AFF: return launch_eo<T, N, Q, OP_KPM_AFF>(r);
+      }
+    }
+    return FEMTC_ERR_INVALID_ARGUMENT;
+  }
   if (r.assembled)
   {
     if (!g || r.reference_mode || r.blocked)

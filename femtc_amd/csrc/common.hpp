@@ -70,6 +70,9 @@ struct ApplyReq
   // overwrite, else y += sum), assembled = 2 stops after the cell pass.
   int assembled = 0;
   void* lattice = nullptr;
+  // Even-odd table set (even_odd.hpp) of symmetric tables, used instead of
+  // tab when set: Q == N, gather, correct mode, direct or assembled scatter.
+  const void* eo_tab = nullptr;
 };
 
 // BW template value of the assembled scatter's cell-pass instances.
@@ -157,5 +160,12 @@ int fill_zero(void* y, int64_t n, int dtype, hipStream_t s);
 // which the assembled scatter requires.  Synchronous on `s` (plan.hip).
 int verify_structured(const int32_t* dofmap, int nx, int ny, int nz,
                       int degree, int64_t ndofs, hipStream_t s, int* ok);
+
+// Reads the device tables [phi | phi'] ([2][N][Q] in `dtype`) on `s`; when
+// they pass eo_symmetric, allocates and fills *eo with their even-odd table
+// set (even_odd.hpp) in `out_dtype`, with the weights w (may be null),
+// else leaves *eo null.  Synchronous on `s` (plan.hip).
+int make_even_odd(const void* tables, int dtype, int N, int Q, const double* w,
+                  int out_dtype, hipStream_t s, void** eo);
 
 } // namespace femtc

@@ -5,11 +5,13 @@
 // offered when the dofmap is verified to be the structured numbering at
 // plan creation), chosen per (nq, dtype, op) by default tables or by timing
 // the candidates on the caller's own data (femtc_plan_autotune).
+#include <cfloat>
 #include <cstdio>
 #include <vector>
 
 #include "assemble.hpp"
 #include "common.hpp"
+#include "even_odd.hpp"
 
 namespace femtc
 {
@@ -180,6 +182,51 @@ int verify_structured(const int32_t* dofmap, int nx, int ny, int nz,
   return st;
 }
 
+int make_even_odd(const void* tables, int dtype, int N, int Q, const double* w,
+                  int out_dtype, hipStream_t s, void** eo)
+{
+  *eo = nullptr;
+  const bool d64 = dtype == FEMTC_F64;
+  const int nt = 2 * N * Q;
+  std::vector<double> t(nt);
+  std::vector<float> t32(d64 ? 0 : nt);
+  void* host = d64 ? (void*)t.data() : (void*)t32.data();
+  if (hipMemcpyAsync(host, tables, (size_t)nt * (d64 ? 8 : 4),
+                     hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return fail(FEMTC_ERR_HIP, std::string("table read: ") +
+                                   hipGetErrorString(hipGetLastError()));
+  if (!d64)
+    for (int i = 0; i < nt; ++i)
+      t[i] = t32[i];
+  if (!eo_symmetric(t.data(), N, Q, d64 ? DBL_EPSILON : FLT_EPSILON))
+    return FEMTC_OK;
+  const int ne = eo_table_size(N, Q);
+  std::vector<double> e(ne);
+  eo_build(t.data(), w, N, Q, e.data());
+  const bool o64 = out_dtype == FEMTC_F64;
+  std::vector<float> e32(o64 ? 0 : ne);
+  for (int i = 0; i < (o64 ? 0 : ne); ++i)
+    e32[i] = (float)e[i];
+  const size_t bytes = (size_t)ne * (o64 ? 8 : 4);
+  void* buf = nullptr;
+  if (hipMalloc(&buf, bytes) != hipSuccess)
+  {
+    (void)hipGetLastError();
+    return fail(FEMTC_ERR_OUT_OF_MEMORY, "even-odd table allocation failed");
+  }
+  if (hipMemcpyAsync(buf, o64 ? (const void*)e.data() : (const void*)e32.data(),
+                     bytes, hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+  {
+    (void)hipFree(buf);
+    return fail(FEMTC_ERR_HIP, std::string("even-odd table upload: ") +
+                                   hipGetErrorString(hipGetLastError()));
+  }
+  *eo = buf;
+  return FEMTC_OK;
+}
+
 } // namespace femtc
 
 using namespace femtc;
@@ -198,6 +245,23 @@ struct femtc_plan
   // chosen block shape per (nq, dtype, op); x == 0 means direct scatter,
   // x < 0 the assembled scatter
   std::vector<std::pair<int, Shape>> shapes;
+  // tables registered by femtc_plan_set_tables, per (dtype, nq): a copy,
+  // and its even-odd table set when the tables are symmetric (else null)
+  struct Tables
+  {
+    int dtype, nq;
+    void* tab;
+    void* eo;
+  };
+  std::vector<Tables> tables;
+
+  const Tables* find_tables(int dtype, int nq) const
+  {
+    for (auto& t : tables)
+      if (t.dtype == dtype && t.nq == nq)
+        return &t;
+    return nullptr;
+  }
 
   Shape shape(int nq, int dtype, int op) const
   {
@@ -223,10 +287,28 @@ struct femtc_plan
 namespace
 {
 
+// The tables of an apply: the caller's, or (tables == NULL) the copy that
+// femtc_plan_set_tables registered, with its even-odd set (*eo).
+int resolve_tables(const femtc_plan* pl, int dtype, int nq, const void** tables,
+                   const void** eo)
+{
+  *eo = nullptr;
+  if (*tables)
+    return FEMTC_OK;
+  const femtc_plan::Tables* t = pl->find_tables(dtype, nq);
+  if (!t)
+    return fail(FEMTC_ERR_INVALID_ARGUMENT,
+                "tables is NULL and none are registered for this dtype and nq "
+                "(femtc_plan_set_tables)");
+  *tables = t->tab;
+  *eo = t->eo;
+  return FEMTC_OK;
+}
+
 int plan_request(const femtc_plan* pl, int op, int dtype, int nq,
-                 const void* tables, const void* x, void* y, const void* G,
-                 const void* detJw, int overwrite, Shape s, hipStream_t st,
-                 ApplyReq* out)
+                 const void* tables, const void* eo, const void* x, void* y,
+                 const void* G, const void* detJw, int overwrite, Shape s,
+                 hipStream_t st, ApplyReq* out)
 {
   ApplyReq r;
   r.op = op;
@@ -235,6 +317,8 @@ int plan_request(const femtc_plan* pl, int op, int dtype, int nq,
   r.Q = nq;
   r.gather = 1;
   r.tab = tables;
+  // the even-odd instances cover the direct and assembled scatters
+  r.eo_tab = s.x <= 0 && nq == r.N ? eo : nullptr;
   r.x = x;
   r.y = y;
   r.G = G;
@@ -280,7 +364,7 @@ int check_apply_args(const femtc_plan* pl, int op, int dtype, int nq,
     return st;
   if (op < FEMTC_OP_MASS || op > FEMTC_OP_KPM)
     return fail(FEMTC_ERR_INVALID_ARGUMENT, "op must be a femtc_op");
-  if (!tables || !x || !y)
+  if (!x || !y)
     return fail(FEMTC_ERR_INVALID_ARGUMENT, "missing buffer");
   if (op != FEMTC_OP_MASS && !G)
     return fail(FEMTC_ERR_INVALID_ARGUMENT, "G is required for this op");
@@ -313,8 +397,9 @@ int ensure_lattice(femtc_plan* pl, int dtype)
 }
 
 int launch_plan(const femtc_plan* pl, int op, int dtype, int nq,
-                const void* tables, const void* x, void* y, const void* G,
-                const void* detJw, int accumulate, Shape s, hipStream_t st)
+                const void* tables, const void* eo, const void* x, void* y,
+                const void* G, const void* detJw, int accumulate, Shape s,
+                hipStream_t st)
 {
   if (s.x < 0 && !pl->lattice[dtype == FEMTC_F64 ? 0 : 1])
     return fail(FEMTC_ERR_INVALID_ARGUMENT,
@@ -326,8 +411,8 @@ int launch_plan(const femtc_plan* pl, int op, int dtype, int nq,
       return r0;
   }
   ApplyReq r;
-  plan_request(pl, op, dtype, nq, tables, x, y, G, detJw, !accumulate, s, st,
-               &r);
+  plan_request(pl, op, dtype, nq, tables, eo, x, y, G, detJw, !accumulate, s,
+               st, &r);
   return run(r);
 }
 
@@ -418,6 +503,12 @@ int femtc_plan_destroy(femtc_plan* plan)
     for (void* b : plan->lattice)
       if (b)
         (void)hipFree(b);
+    for (auto& t : plan->tables)
+    {
+      (void)hipFree(t.tab);
+      if (t.eo)
+        (void)hipFree(t.eo);
+    }
     delete plan;
   }
   return FEMTC_OK;
@@ -472,10 +563,63 @@ int femtc_plan_apply(const femtc_plan* plan, int op, int dtype, int nq,
                      void* stream)
 {
   int st = check_apply_args(plan, op, dtype, nq, tables, x, y, G, detJw);
+  const void* eo = nullptr;
+  if (st == FEMTC_OK)
+    st = resolve_tables(plan, dtype, nq, &tables, &eo);
   if (st != FEMTC_OK)
     return st;
-  return launch_plan(plan, op, dtype, nq, tables, x, y, G, detJw, accumulate,
-                     plan->shape(nq, dtype, op), (hipStream_t)stream);
+  return launch_plan(plan, op, dtype, nq, tables, eo, x, y, G, detJw,
+                     accumulate, plan->shape(nq, dtype, op),
+                     (hipStream_t)stream);
+}
+
+int femtc_plan_set_tables(femtc_plan* plan, int dtype, int nq,
+                          const void* tables, void* stream, int* even_odd)
+{
+  if (even_odd)
+    *even_odd = 0;
+  if (!plan || !tables)
+    return fail(FEMTC_ERR_INVALID_ARGUMENT, "plan and tables are required");
+  int st = check_shape(plan->degree, nq, FEMTC_MODE_CORRECT);
+  if (st == FEMTC_OK)
+    st = check_dtype(dtype);
+  if (st != FEMTC_OK)
+    return st;
+  hipStream_t s = (hipStream_t)stream;
+  const int N = plan->degree + 1;
+  const size_t bytes = (size_t)2 * N * nq * (dtype == FEMTC_F64 ? 8 : 4);
+  void* copy = nullptr;
+  void* eo = nullptr;
+  if (hipMalloc(&copy, bytes) != hipSuccess)
+  {
+    (void)hipGetLastError();
+    return fail(FEMTC_ERR_OUT_OF_MEMORY, "table copy allocation failed");
+  }
+  if (hipMemcpyAsync(copy, tables, bytes, hipMemcpyDeviceToDevice, s) !=
+      hipSuccess)
+    st = fail(FEMTC_ERR_HIP, "table copy failed");
+  if (st == FEMTC_OK && nq == N)
+    st = make_even_odd(copy, dtype, N, nq, nullptr, dtype, s, &eo);
+  else if (st == FEMTC_OK && hipStreamSynchronize(s) != hipSuccess)
+    st = fail(FEMTC_ERR_HIP, "table copy failed");
+  if (st != FEMTC_OK)
+  {
+    (void)hipFree(copy);
+    return st;
+  }
+  for (auto it = plan->tables.begin(); it != plan->tables.end(); ++it)
+    if (it->dtype == dtype && it->nq == nq)
+    {
+      (void)hipFree(it->tab);
+      if (it->eo)
+        (void)hipFree(it->eo);
+      plan->tables.erase(it);
+      break;
+    }
+  plan->tables.push_back({dtype, nq, copy, eo});
+  if (even_odd)
+    *even_odd = eo ? 1 : 0;
+  return FEMTC_OK;
 }
 
 int femtc_plan_autotune(femtc_plan* plan, int op, int dtype, int nq,
@@ -484,6 +628,9 @@ int femtc_plan_autotune(femtc_plan* plan, int op, int dtype, int nq,
                         void* stream, double* best_us)
 {
   int st = check_apply_args(plan, op, dtype, nq, tables, x, y, G, detJw);
+  const void* eo = nullptr;
+  if (st == FEMTC_OK)
+    st = resolve_tables(plan, dtype, nq, &tables, &eo);
   if (st != FEMTC_OK)
     return st;
   if (reps < 1)
@@ -503,14 +650,14 @@ int femtc_plan_autotune(femtc_plan* plan, int op, int dtype, int nq,
     // warm-up, then time reps overwriting applies (each scatter with the
     // clearing of y it needs: a fill kernel for the atomic and blocked
     // ones, none for the assembled one)
-    st = launch_plan(plan, op, dtype, nq, tables, x, y, G, detJw, 0, c, s);
+    st = launch_plan(plan, op, dtype, nq, tables, eo, x, y, G, detJw, 0, c, s);
     if (st == FEMTC_ERR_INVALID_ARGUMENT)
       continue; // shape does not fit (LDS) for this op/dtype
     if (st != FEMTC_OK)
       break;
     (void)hipEventRecord(e0, s);
     for (int i = 0; i < reps && st == FEMTC_OK; ++i)
-      st = launch_plan(plan, op, dtype, nq, tables, x, y, G, detJw, 0, c, s);
+      st = launch_plan(plan, op, dtype, nq, tables, eo, x, y, G, detJw, 0, c, s);
     (void)hipEventRecord(e1, s);
     if (st != FEMTC_OK || hipEventSynchronize(e1) != hipSuccess)
       break;

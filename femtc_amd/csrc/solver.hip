@@ -598,6 +598,12 @@ struct femtc_solver
   // dofmap): its lattice buffer in the inner precision
   bool inner_assembled = false;
   void* lattice = nullptr;
+  // even-odd table sets (symmetric tables, nq = degree + 1): FP64 for the
+  // FP64 operators, inner precision for the inner one (aliases eo64 for an
+  // FP64 inner solve)
+  bool even_odd = false;
+  void* eo64 = nullptr;
+  void* eo_in = nullptr;
   // the batch of check_every iterations as a hipGraph (captured on
   // cap_stream at the first solve, replayed on `stream`)
   bool use_graph = true;
@@ -661,6 +667,9 @@ void destroy(femtc_solver* S)
     dev_free(S->G);
     dev_free(S->detJw);
   }
+  if (S->eo_in != S->eo64)
+    dev_free(S->eo_in);
+  dev_free(S->eo64);
   dev_free(S->tabw64);
   dev_free(S->cg64);
   dev_free(S->epart);
@@ -695,6 +704,8 @@ ApplyReq operator_req(const femtc_solver* S, int dtype, const void* tab,
   a.dofmap = S->prob.dofmap;
   a.num_cells = S->prob.num_cells;
   a.stream = S->stream;
+  if (S->even_odd)
+    a.eo_tab = dtype == FEMTC_F64 ? S->eo64 : S->eo_in;
   return a;
 }
 
@@ -812,50 +823,72 @@ int time_step(femtc_solver* S, int reps, float* ms)
   return st;
 }
 
-// Scatter of the inner operator from problem.grid/scatter: the assembled
-// one when asked for, or, under FEMTC_SCATTER_AUTO, when its iteration step
-// times faster than the atomic one on this problem (vectors zeroed and
+// Scatter and contraction form of the operators.  The scatters allowed by
+// problem.grid/scatter (the assembled one only for the structured dofmap;
+// FEMTC_SCATTER_AUTO allows both) and the forms available (even-odd when
+// the tables passed the check) are each timed over one CG iteration step
+// on this problem, and the fastest pair is kept (vectors zeroed and
 // alpha = 0 for the timing; the solve re-initialises everything).
-int choose_scatter(femtc_solver* S, int structured)
+int choose_variant(femtc_solver* S, int structured)
 {
   const int mode = S->prob.scatter;
-  if (!structured || mode == FEMTC_SCATTER_DIRECT)
-    return FEMTC_OK;
-  const size_t es = S->inner_dtype == FEMTC_F64 ? 8 : 4;
-  if (dev_alloc(&S->lattice,
-                es * (size_t)lattice_size(S->prob.grid[0], S->prob.grid[1],
-                                          S->prob.grid[2], S->N - 1)))
+  const bool have_eo = S->eo64 && S->eo_in;
+  bool try_asm = structured && mode != FEMTC_SCATTER_DIRECT;
+  if (try_asm)
   {
-    if (mode == FEMTC_SCATTER_ASSEMBLED)
-      return FEMTC_ERR_OUT_OF_MEMORY;
-    return FEMTC_OK; // no room: keep the atomic scatter
+    const size_t es = S->inner_dtype == FEMTC_F64 ? 8 : 4;
+    if (dev_alloc(&S->lattice,
+                  es * (size_t)lattice_size(S->prob.grid[0], S->prob.grid[1],
+                                            S->prob.grid[2], S->N - 1)))
+    {
+      if (mode == FEMTC_SCATTER_ASSEMBLED)
+        return FEMTC_ERR_OUT_OF_MEMORY;
+      try_asm = false; // no room: keep the atomic scatter
+    }
   }
-  if (mode == FEMTC_SCATTER_ASSEMBLED)
-  {
-    S->inner_assembled = true;
-    return FEMTC_OK;
-  }
-  hipStream_t s = S->stream;
-  const int dt = S->inner_dtype;
+  const bool try_direct = !(try_asm && mode == FEMTC_SCATTER_ASSEMBLED);
+  // even-odd forms to try: none, the even-odd one only (FEMTC_EVEN_ODD_ON),
+  // or both
+  const int e0 = have_eo && S->prob.even_odd == FEMTC_EVEN_ODD_ON ? 1 : 0;
+  const int e1 = have_eo ? 2 : 1;
+  std::vector<std::pair<bool, bool>> cand; // (assembled, even-odd)
+  for (int a = 0; a < 2; ++a)
+    if (a ? try_asm : try_direct)
+      for (int e = e0; e < e1; ++e)
+        cand.push_back({a == 1, e == 1});
+  size_t best = 0;
   int st = FEMTC_OK;
-  for (void* v : {S->e, S->r, S->p, S->Ap})
-    if (st == FEMTC_OK)
-      st = fill_zero(v, S->n, dt, s);
-  PcgState h{};
-  if (st == FEMTC_OK &&
-      (hipMemcpyAsync(S->st, &h, sizeof(h), hipMemcpyHostToDevice, s) !=
-           hipSuccess ||
-       hipStreamSynchronize(s) != hipSuccess))
-    st = fail(FEMTC_ERR_HIP, "solver: state upload failed");
-  float t[2] = {0.f, 0.f};
-  for (int c = 0; c < 2 && st == FEMTC_OK; ++c)
+  if (cand.size() > 1)
   {
-    S->inner_assembled = c == 1;
-    st = dt == FEMTC_F64 ? time_step<double>(S, 3, &t[c])
-                         : time_step<float>(S, 3, &t[c]);
+    hipStream_t s = S->stream;
+    const int dt = S->inner_dtype;
+    for (void* v : {S->e, S->r, S->p, S->Ap})
+      if (st == FEMTC_OK)
+        st = fill_zero(v, S->n, dt, s);
+    PcgState h{};
+    if (st == FEMTC_OK &&
+        (hipMemcpyAsync(S->st, &h, sizeof(h), hipMemcpyHostToDevice, s) !=
+             hipSuccess ||
+         hipStreamSynchronize(s) != hipSuccess))
+      st = fail(FEMTC_ERR_HIP, "solver: state upload failed");
+    float tbest = 0.f;
+    for (size_t c = 0; c < cand.size() && st == FEMTC_OK; ++c)
+    {
+      S->inner_assembled = cand[c].first;
+      S->even_odd = cand[c].second;
+      float t = 0.f;
+      st = dt == FEMTC_F64 ? time_step<double>(S, 3, &t)
+                           : time_step<float>(S, 3, &t);
+      if (c == 0 || t < tbest)
+      {
+        tbest = t;
+        best = c;
+      }
+    }
   }
-  S->inner_assembled = st == FEMTC_OK && t[1] < t[0];
-  if (!S->inner_assembled)
+  S->inner_assembled = cand[best].first;
+  S->even_odd = cand[best].second;
+  if (!S->inner_assembled && S->lattice)
   {
     dev_free(S->lattice);
     S->lattice = nullptr;
@@ -1056,6 +1089,9 @@ int check_problem(const femtc_problem* P)
   if (P->scatter == FEMTC_SCATTER_ASSEMBLED && nogrid)
     return fail(FEMTC_ERR_INVALID_ARGUMENT,
                 "the assembled scatter needs the structured grid");
+  if (P->even_odd < FEMTC_EVEN_ODD_AUTO || P->even_odd > FEMTC_EVEN_ODD_ON)
+    return fail(FEMTC_ERR_INVALID_ARGUMENT,
+                "even_odd must be FEMTC_EVEN_ODD_AUTO, _OFF or _ON");
   return FEMTC_OK;
 }
 
@@ -1249,7 +1285,23 @@ int femtc_solver_create(femtc_solver** out, const femtc_problem* prob,
                 "no cell touches, or geometry that is not positive definite)");
       break;
     }
-    // scatter of the inner operator
+    // even-odd table sets when the tables are symmetric (the weights of the
+    // Gauss rule ride along for the per-cell geometry operators); whether
+    // they are used is timed below with the scatter
+    if (prob->even_odd != FEMTC_EVEN_ODD_OFF && S->Q == S->N)
+    {
+      std::vector<double> xq(S->Q), wq(S->Q);
+      femtc_gauss_rule(S->Q, xq.data(), wq.data());
+      if ((st = make_even_odd(prob->tables, FEMTC_F64, S->N, S->Q, wq.data(),
+                              FEMTC_F64, s, &S->eo64)) != FEMTC_OK)
+        break;
+      S->eo_in = S->eo64;
+      if (S->eo64 && inner_dtype != FEMTC_F64 &&
+          (st = make_even_odd(prob->tables, FEMTC_F64, S->N, S->Q, wq.data(),
+                              inner_dtype, s, &S->eo_in)) != FEMTC_OK)
+        break;
+    }
+    // scatter and contraction form of the operators
     int structured = 0;
     if (prob->grid[0] > 0 &&
         (st = verify_structured(prob->dofmap, prob->grid[0], prob->grid[1],
@@ -1263,7 +1315,7 @@ int femtc_solver_create(femtc_solver** out, const femtc_problem* prob,
                 "numbering of the grid");
       break;
     }
-    if ((st = choose_scatter(S, structured)) != FEMTC_OK)
+    if ((st = choose_variant(S, structured)) != FEMTC_OK)
       break;
   } while (false);
   if (st != FEMTC_OK)
@@ -1317,6 +1369,14 @@ int femtc_solver_scatter(const femtc_solver* solver, int* scatter)
     return fail(FEMTC_ERR_INVALID_ARGUMENT, "NULL argument");
   *scatter = solver->inner_assembled ? FEMTC_SCATTER_ASSEMBLED
                                      : FEMTC_SCATTER_DIRECT;
+  return FEMTC_OK;
+}
+
+int femtc_solver_even_odd(const femtc_solver* solver, int* even_odd)
+{
+  if (!solver || !even_odd)
+    return fail(FEMTC_ERR_INVALID_ARGUMENT, "NULL argument");
+  *even_odd = solver->even_odd ? 1 : 0;
   return FEMTC_OK;
 }
 

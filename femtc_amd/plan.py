@@ -38,6 +38,15 @@ class Plan:
             "plan_get_block")
         return (a.value, b.value, c.value)
 
+    def set_tables(self, tables, nq: int, stream=None) -> bool:
+        """Register a copy of `tables` (its dtype) for applies passing
+        tables=None; True when they contract in even-odd form."""
+        eo = C.c_int32()
+        _lib.check(_lib.lib().femtc_plan_set_tables(
+            self._h, _dtype_code(tables), nq, _ptr(tables), _stream(stream),
+            C.byref(eo)), "plan_set_tables")
+        return bool(eo.value)
+
     def apply(self, op: int, tables, x, y, nq: int, G=None, detJw=None,
               accumulate: bool = False, stream=None) -> None:
         _lib.check(_lib.lib().femtc_plan_apply(

@@ -33,13 +33,17 @@ class PCGSolver:
     def __init__(self, op: int, tables, dofmap, num_cells: int, degree: int,
                  nq: int, ndofs: int, G=None, detJw=None, inner: int = _lib.F64,
                  stream=None, J=None, grid=None,
-                 scatter: int = _lib.SCATTER_AUTO):
+                 scatter: int = _lib.SCATTER_AUTO, even_odd=None):
         """J: optional per-cell Jacobians [C*9] (FP64) of affine cells; the
         inner operator then computes its geometry in the kernel.
         grid: optional (nx, ny, nz) of a structured cell numbering; when the
         dofmap is that numbering the CG iterations can use the assembled
         scatter (scatter: SCATTER_AUTO times it against the atomic one,
-        SCATTER_DIRECT / SCATTER_ASSEMBLED force one)."""
+        SCATTER_DIRECT / SCATTER_ASSEMBLED force one).
+        even_odd: None times the even-odd form (nq = degree + 1 and
+        symmetric tables, checked once, here) against the general one and
+        keeps the faster; True uses it whenever the check passes; False
+        keeps the general form."""
         self._keep = (tables, dofmap, G, detJw, J)
         self.ndofs = ndofs
         self.inner = inner
@@ -47,7 +51,8 @@ class PCGSolver:
         self.problem = _lib.Problem(op, degree, nq, num_cells, ndofs,
                                     _ptr(dofmap), _ptr(tables), _ptr(G),
                                     _ptr(detJw), _ptr(J), (C.c_int32 * 3)(*g),
-                                    scatter)
+                                    scatter,
+                                    {None: 0, False: 1, True: 2}[even_odd])
         h = C.c_void_p()
         _lib.check(_lib.lib().femtc_solver_create(
             C.byref(h), C.byref(self.problem), inner, _stream(stream)),
@@ -68,6 +73,13 @@ class PCGSolver:
         _lib.check(_lib.lib().femtc_solver_scatter(self._h, C.byref(a)),
                    "solver_scatter")
         return a.value
+
+    def even_odd(self) -> bool:
+        """Whether the solver's operators contract in even-odd form."""
+        a = C.c_int32()
+        _lib.check(_lib.lib().femtc_solver_even_odd(self._h, C.byref(a)),
+                   "solver_even_odd")
+        return bool(a.value)
 
     def solve(self, f, u, rtol: float = 1e-10, inner_rtol: float = 1e-4,
               maxit: int = 20000, check: bool = True) -> dict:

@@ -243,11 +243,25 @@ int femtc_plan_set_block(femtc_plan* plan, int nq, int dtype, int op, int cbx,
                          int cby, int cbz);
 int femtc_plan_get_block(const femtc_plan* plan, int nq, int dtype, int op,
                          int* cbx, int* cby, int* cbz);
-/* y = A x (accumulate = 0: y overwritten) or y += A x (accumulate = 1). */
+/* y = A x (accumulate = 0: y overwritten) or y += A x (accumulate = 1).
+ * tables may be NULL when femtc_plan_set_tables registered them for
+ * (dtype, nq). */
 int femtc_plan_apply(const femtc_plan* plan, int op, int dtype, int nq,
                      const void* tables, const void* x, void* y,
                      const void* G, const void* detJw, int accumulate,
                      void* stream);
+/* Registers a copy of the tables [phi | dphi] ([2][degree+1][nq], dtype;
+ * taken now, synchronously on `stream`) for the applies and autotuning of
+ * (dtype, nq) that pass tables = NULL; a second call replaces it.  When
+ * nq = degree + 1 and the tables are those of a symmetric node set and
+ * rule (phi symmetric and dphi antisymmetric under b -> N-1-b,
+ * q -> nq-1-q, to 64 rounding units, as femtc_basis_tables makes them),
+ * those applies contract each line in even-odd form (half-size products;
+ * results equal to rounding) on the direct and assembled scatters;
+ * *even_odd (may be NULL) reports whether they do.  No counterpart in the
+ * reference. */
+int femtc_plan_set_tables(femtc_plan* plan, int dtype, int nq,
+                          const void* tables, void* stream, int* even_odd);
 /* Time every candidate block shape (and the direct and, for the structured
  * dofmap, the assembled scatter) on these buffers, reps overwriting applies
  * each (y = A x, clearing included), and keep the fastest for
@@ -358,7 +372,22 @@ typedef struct femtc_problem
    * scatter: */
   int grid[3];
   int scatter; /* femtc_scatter */
+  /* FEMTC_EVEN_ODD_AUTO (0): when nq = degree + 1 and the tables pass the
+   * symmetry check of femtc_plan_set_tables (they are read once, at
+   * creation), the even-odd form of the operators is timed over one CG
+   * iteration step against the general one, with each scatter the solver
+   * may use, and the fastest pair is kept.  FEMTC_EVEN_ODD_OFF (1): the
+   * general form only.  FEMTC_EVEN_ODD_ON (2): the even-odd form whenever
+   * the tables pass the check (not timed). */
+  int even_odd;
 } femtc_problem;
+
+enum
+{
+  FEMTC_EVEN_ODD_AUTO = 0,
+  FEMTC_EVEN_ODD_OFF = 1,
+  FEMTC_EVEN_ODD_ON = 2
+};
 
 typedef enum femtc_scatter
 {
@@ -404,6 +433,8 @@ int femtc_solver_diagonal(const femtc_solver* solver, const double** diag);
 /* The scatter the solver chose for its CG iterations
  * (FEMTC_SCATTER_DIRECT or FEMTC_SCATTER_ASSEMBLED). */
 int femtc_solver_scatter(const femtc_solver* solver, int* scatter);
+/* 1 when the solver's operators contract in even-odd form, else 0. */
+int femtc_solver_even_odd(const femtc_solver* solver, int* even_odd);
 /* CG iterations queued between two reads of the convergence state
  * (check_every, default 16; iterations past convergence cost a kernel
  * launch each and do no work), and whether that batch is replayed from a

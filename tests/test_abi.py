@@ -139,10 +139,17 @@ def test_solver_create_rejects_bad_problems_without_touching_the_device():
     assert L.femtc_solver_create(C_.byref(h), C_.byref(p), F.F64, None) == \
         _lib.ERR_INVALID_ARGUMENT
     assert "femtc_scatter" in _lib.last_error()
+    p = _lib.Problem(F.OP_KPM, 3, 4, 10, 100, 1, 1, 1, 1, None, grid(1, 2, 5),
+                     F.SCATTER_AUTO, 3)
+    assert L.femtc_solver_create(C_.byref(h), C_.byref(p), F.F64, None) == \
+        _lib.ERR_INVALID_ARGUMENT
+    assert "even_odd" in _lib.last_error()
     assert h.value is None
     assert L.femtc_solver_set_batch(None, 16, 1) == _lib.ERR_INVALID_ARGUMENT
     s = C_.c_int32()
     assert L.femtc_solver_scatter(None, C_.byref(s)) == _lib.ERR_INVALID_ARGUMENT
+    assert L.femtc_solver_even_odd(None, C_.byref(s)) == \
+        _lib.ERR_INVALID_ARGUMENT
 
 
 def test_plan_create_validates_arguments_without_touching_the_device():
@@ -158,6 +165,10 @@ def test_plan_create_validates_arguments_without_touching_the_device():
     assert L.femtc_plan_create(None, 2, 2, 2, 3, 1, 100, None) == \
         _lib.ERR_INVALID_ARGUMENT
     assert h.value is None
+    eo = C_.c_int32(7)
+    assert L.femtc_plan_set_tables(None, F.F64, 4, 1, None, C_.byref(eo)) == \
+        _lib.ERR_INVALID_ARGUMENT
+    assert eo.value == 0
     assert L.femtc_plan_apply(None, 0, 0, 4, 1, 1, 1, None, 1, 0, None) == \
         _lib.ERR_INVALID_ARGUMENT
 

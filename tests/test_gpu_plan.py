@@ -221,3 +221,85 @@ def test_plan_rejects_bad_input():
     plan = F.Plan(2, 2, 2, P, dev(dm, torch.int32), ng)
     with pytest.raises(F.FemtcError, match="multiple"):
         plan.set_block(Q, F.F64, F.OP_MASS, (2, 2, 3))  # cbz % 7 != 0
+
+
+# (nx, ny, nz, P, Q = P + 1): even and odd N, every degree
+EO_MESHES = [(3, 5, 5, 1, 2), (3, 5, 9, 2, 3), (5, 3, 6, 3, 4), (3, 5, 5, 4, 5),
+             (3, 3, 4, 5, 6), (3, 3, 3, 6, 7), (3, 2, 3, 7, 8), (2, 2, 3, 8, 9)]
+
+
+@pytest.mark.parametrize("nx,ny,nz,P,Q", EO_MESHES)
+@pytest.mark.parametrize("op", [F.OP_MASS, F.OP_STIFFNESS, F.OP_KPM])
+def test_plan_even_odd_vs_oracle(op, nx, ny, nz, P, Q):
+    # registered symmetric tables: the even-odd contractions on the direct
+    # and the assembled scatter, FP64 and FP32, overwrite and accumulate,
+    # against the oracle and against the general form on the same plan
+    C, G, Dw, dm, ng = _box(nx, ny, nz, P, Q, 150 + P)
+    tab = O.basis_tables(P, Q)
+    x = O.uniform(151, ng)
+    y0 = O.uniform(152, ng)
+    ref = O.global_operator(op, P + 1, Q, tab, x, dm, G, Dw)
+    plan = F.Plan(nx, ny, nz, P, dev(dm, torch.int32), ng)
+    for dt, code, tol, tol_gen in ((torch.float64, F.F64, 1e-12, 1e-13),
+                                   (torch.float32, F.F32, 1e-5, 2e-6)):
+        assert plan.set_tables(dev(tab, dt), Q) is True
+        dtab, dx, dG, dD = dev(tab, dt), dev(x, dt), dev(G, dt), dev(Dw, dt)
+        for shape in ((0, 0, 0), (-1, -1, -1)):
+            plan.set_block(Q, code, op, shape)
+            dy = torch.full((ng,), 5.0, device="cuda", dtype=dt)
+            plan.apply(op, None, dx, dy, Q, G=dG, detJw=dD)
+            assert rel(host(dy), ref) <= tol, (dt, shape)
+            dg = torch.full((ng,), 5.0, device="cuda", dtype=dt)
+            plan.apply(op, dtab, dx, dg, Q, G=dG, detJw=dD)
+            assert rel(host(dy), host(dg)) <= tol_gen, (dt, shape)
+            dy = dev(y0, dt)
+            plan.apply(op, None, dx, dy, Q, G=dG, detJw=dD, accumulate=True)
+            assert rel(host(dy), ref + y0) <= tol, (dt, shape)
+
+
+def test_plan_registered_tables_without_symmetry_use_the_general_form():
+    nx, ny, nz, P = 3, 3, 4, 4
+    plan_dm = O.dofmap_structured(nx, ny, nz, P)
+    ng = (nx * P + 1) * (ny * P + 1) * (nz * P + 1)
+    plan = F.Plan(nx, ny, nz, P, dev(plan_dm, torch.int32), ng)
+    x = O.uniform(161, ng)
+    # unregistered (dtype, nq): tables=None is refused
+    dy = torch.empty(ng, device="cuda", dtype=torch.float64)
+    with pytest.raises(F.FemtcError, match="registered"):
+        plan.apply(F.OP_MASS, None, dev(x), dy, P + 1, detJw=dev(np.ones(1)))
+    for Q, perturb in ((P + 1, True), (P + 2, False)):
+        C, G, Dw, dm, _ = _box(nx, ny, nz, P, Q, 160)
+        tab = O.basis_tables(P, Q)
+        if perturb:
+            tab = tab.copy()
+            tab[1] *= 1.0 + 1e-9  # breaks phi's symmetry by 1e-9
+        ref = O.global_operator(F.OP_KPM, P + 1, Q, tab, x, dm, G, Dw)
+        assert plan.set_tables(dev(tab), Q) is False
+        for shape in ((0, 0, 0), (-1, -1, -1)):
+            plan.set_block(Q, F.F64, F.OP_KPM, shape)
+            plan.apply(F.OP_KPM, None, dev(x), dy, Q, G=dev(G), detJw=dev(Dw))
+            assert rel(host(dy), ref) <= 1e-12, (Q, shape)
+    # registering again replaces the copy (now symmetric: even-odd)
+    Q = P + 1
+    C, G, Dw, dm, _ = _box(nx, ny, nz, P, Q, 160)
+    tab = O.basis_tables(P, Q)
+    assert plan.set_tables(dev(tab), Q) is True
+    ref = O.global_operator(F.OP_KPM, P + 1, Q, tab, x, dm, G, Dw)
+    plan.apply(F.OP_KPM, None, dev(x), dy, Q, G=dev(G), detJw=dev(Dw))
+    assert rel(host(dy), ref) <= 1e-12
+
+
+def test_plan_autotune_with_registered_tables():
+    nx, ny, nz, P, Q = 6, 6, 6, 4, 5
+    C, G, Dw, dm, ng = _box(nx, ny, nz, P, Q, 170)
+    tab = O.basis_tables(P, Q)
+    x = O.uniform(171, ng)
+    ref = O.global_operator(F.OP_STIFFNESS, P + 1, Q, tab, x, dm, G, None)
+    plan = F.Plan(nx, ny, nz, P, dev(dm, torch.int32), ng)
+    assert plan.set_tables(dev(tab), Q) is True
+    dx, dG = dev(x), dev(G)
+    dy = torch.full((ng,), 1.0, device="cuda", dtype=torch.float64)
+    shape, us = plan.autotune(F.OP_STIFFNESS, None, dx, dy, Q, G=dG, reps=2)
+    assert us > 0 and float(dy.abs().max()) == 0.0
+    plan.apply(F.OP_STIFFNESS, None, dx, dy, Q, G=dG)
+    assert rel(host(dy), ref) <= 1e-12, shape

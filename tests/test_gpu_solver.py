@@ -320,3 +320,58 @@ def test_refined_mid_size_p6_properties():
     F.global_apply(F.OP_KPM, tab, u32, y, ddm, C, P, Q, G=dG, detJw=dD)
     assert float((f - y).norm() / f.norm()) <= 1.05e-10
     assert float((u64 - u32).norm() / u64.norm()) <= 1e-8
+
+
+# (op, nx, ny, nz, P, Q, geometry): "point" streams G/detJw, "cell" gives J
+# as well (per-cell inner operator), "cell only" gives J alone (every
+# operator, the FP64 residual included, computes its geometry per cell)
+EO_SOLVES = [(F.OP_KPM, 3, 3, 3, 2, 3, "point"),
+             (F.OP_MASS, 3, 2, 2, 3, 4, "point"),
+             (F.OP_KPM, 2, 3, 2, 4, 5, "cell"),
+             (F.OP_MASS, 2, 2, 3, 5, 6, "cell only"),
+             (F.OP_KPM, 2, 2, 2, 6, 7, "cell only"),
+             (F.OP_KPM, 2, 1, 2, 8, 9, "cell")]
+
+
+@pytest.mark.parametrize("inner", [F.F64, F.F32])
+@pytest.mark.parametrize("op,nx,ny,nz,P,Q,geometry", EO_SOLVES)
+def test_solver_even_odd_and_general_forms(inner, op, nx, ny, nz, P, Q,
+                                           geometry):
+    # the solver's even-odd operators (symmetric tables, nq = P + 1) and the
+    # general ones, on both scatters: every solve meets the tolerance under
+    # the oracle's operator and matches the oracle's solution, and the two
+    # forms drive CG alike (iterations +- 2)
+    seed = 180 + P
+    C, G, Dw, dm, ng, tab, f = _problem(op, nx, ny, nz, P, Q, seed)
+    name = {F.OP_MASS: "mass", F.OP_KPM: "kpm"}[op]
+    u_ref, _, _ = O.pcg(name, P + 1, Q, tab, f, dm, G, Dw, rtol=1e-14)
+    kw = {}
+    if geometry == "point" or geometry == "cell":
+        kw = dict(detJw=dev(Dw), G=dev(G) if op != F.OP_MASS else None)
+    if geometry != "point":
+        kw["J"] = dev(O.random_jacobians(seed, C))
+    iters = {}
+    for even_odd in (True, False):
+        for scatter in (F.SCATTER_DIRECT, F.SCATTER_ASSEMBLED):
+            with F.PCGSolver(op, dev(tab), dev(dm, torch.int32), C, P, Q, ng,
+                             inner=inner, grid=(nx, ny, nz), scatter=scatter,
+                             even_odd=even_odd, **kw) as S:
+                assert S.even_odd() is even_odd
+                u = torch.empty(ng, device="cuda", dtype=torch.float64)
+                info = S.solve(dev(f), u, rtol=1e-10)
+            uh = host(u)
+            res = rel(O.global_operator(op, P + 1, Q, tab, uh, dm, G, Dw), f)
+            key = (even_odd, scatter)
+            assert info["relres"] <= 1.01e-10 and res <= 1.05e-10, (key, info)
+            assert rel(uh, u_ref) <= 1e-8, key
+            iters[key] = info["inner_iterations"]
+    for scatter in (F.SCATTER_DIRECT, F.SCATTER_ASSEMBLED):
+        assert abs(iters[(True, scatter)] - iters[(False, scatter)]) <= 2, iters
+
+
+def test_solver_even_odd_needs_nq_equal_degree_plus_one():
+    op, nx, ny, nz, P, Q = F.OP_KPM, 2, 2, 2, 3, 5
+    C, G, Dw, dm, ng, tab, f = _problem(op, nx, ny, nz, P, Q, 190)
+    with F.PCGSolver(op, dev(tab), dev(dm, torch.int32), C, P, Q, ng,
+                     G=dev(G), detJw=dev(Dw)) as S:
+        assert S.even_odd() is False
