@@ -24,6 +24,8 @@ Beside the metric line's required fields it reports, on the same GPU:
              also with per-cell geometry and with the structured grid given
              (assembled scatter)
   apply_p4_64cube  the P=4 64^3 stiffness apply (GDoF/s, roofline fraction)
+  mass_c1    the cell-local P=3 mass apply on 32768 cells, cache-resident
+             and streamed over 16 rotating buffer sets
   batch      200 uneven meshes in one batched, graph-captured launch set
   serving    a 500 req/s Poisson stream of applies and solves (p50/p99)
 
@@ -309,6 +311,62 @@ def bench_apply_p4(torch, F, steps=30, warmup=60):
                      "roofline_frac": ab / (pms * 1e-3) / 1e9 / HBM_PEAK_GBS}}
 
 
+def bench_mass_c1(torch, F, sets=16, reps=8):
+    """Cell-local mass apply W = M U: 32768 cells, P=3 GLL, Q=5 Gauss, U and
+    W [C][64], detJw [C][125] ~ U[0.5, 1.5].  One set of buffers (66 MB)
+    stays in the 256 MB Infinity Cache, so the kernel is timed both on one
+    set (cache-resident) and rotating over `sets` sets (1.06 GB, streamed
+    from HBM).  The applies are replayed from a captured graph, so the time
+    per apply is the kernel's, without host launch gaps."""
+    dev = torch.device("cuda:0")
+    P, Q, C = 3, 5, 32768
+    N = P + 1
+    tab = torch.from_numpy(F.basis_tables(P, Q)).to(dev)
+    bufs = []
+    for s in range(sets):
+        U = torch.from_numpy(_uniform(100 + s, C * N ** 3, -1, 1)).to(dev)
+        D = torch.from_numpy(_uniform(200 + s, C * Q ** 3, 0.5, 1.5)).to(dev)
+        bufs.append((U, torch.empty_like(U), D))
+
+    def per_apply_us(order):
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):  # warm-up outside the graph
+            for k in order:
+                U, W, D = bufs[k]
+                F.mass_apply(tab, U, W, D, C, P, Q)
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            for k in order:
+                U, W, D = bufs[k]
+                F.mass_apply(tab, U, W, D, C, P, Q)
+        g.replay()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            g.replay()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) * 1e3 / (reps * len(order))
+
+    rot = per_apply_us([i % sets for i in range(4 * sets)])
+    res = per_apply_us([0] * (4 * sets))
+    ab = 8 * C * (2 * N ** 3 + Q ** 3)
+    nd = C * N ** 3
+    return {"workload": "cell-local mass apply W = M U, 32768 cells, P=3 GLL, "
+                        "Q=5 Gauss, FP64",
+            "algorithmic_bytes": ab,
+            "us_per_apply_rotated": rot,
+            "gdofs_rotated": nd / (rot * 1e-6) / 1e9,
+            "roofline_frac_rotated": ab / (rot * 1e-6) / 1e9 / HBM_PEAK_GBS,
+            "us_per_apply_cache_resident": res,
+            "gdofs_cache_resident": nd / (res * 1e-6) / 1e9}
+
+
 def bench_batch(torch, F):
     from femtc_amd import workloads as Wk
     out = {}
@@ -365,8 +423,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--apply-only", action="store_true",
                     help="skip the solve, P=4 apply, batch and serving parts")
-    ap.add_argument("--only", choices=("solve", "apply_p4_64cube", "batch",
-                                       "serving"),
+    ap.add_argument("--only", choices=("solve", "apply_p4_64cube",
+                                       "mass_c1", "batch", "serving"),
                     help="run the metric apply and this part only")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
                     help="only copy MiB three times (counter calibration)")
@@ -444,7 +502,8 @@ def main():
     torch.cuda.empty_cache()
     if not a.apply_only:
         for key, fn in (("solve", bench_solve), ("apply_p4_64cube", bench_apply_p4),
-                        ("batch", bench_batch), ("serving", bench_serving)):
+                        ("mass_c1", bench_mass_c1), ("batch", bench_batch),
+                        ("serving", bench_serving)):
             if a.only and key != a.only:
                 continue
             print(f"[bench] {key}", file=sys.stderr, flush=True)
