@@ -9,7 +9,8 @@ from ._lib import (BF16, F32, F64, MODE_CORRECT, MODE_REFERENCE, OP_KPM,
                    SCATTER_DIRECT, FemtcError, lib)
 from .linalg import Order, string2order
 from .mesh import (basis_tables, basis_tables_weights, dofmap_structured,
-                   gauss_rule, geometry_affine, geometry_cell, gll_points)
+                   even_odd_tables, gauss_rule, geometry_affine,
+                   geometry_cell, gll_points)
 from .operators import (fill_zero, global_apply, global_apply_affine,
                         kpm_apply, mass_apply, mass_operator, stiffness_apply,
                         stiffness_operator)

@@ -68,6 +68,8 @@ SIGNATURES = {
     "femtc_geometry_affine": (_i32, [_i64, _i32, _vp, _vp, _vp, _i32, _vp]),
     "femtc_geometry_cell": (_i32, [_i64, _vp, _vp, _i32, _vp]),
     "femtc_basis_tables_weights": (_i32, [_i32, _i32, _dp]),
+    "femtc_even_odd_tables": (_i32, [_i32, _i32, _dp, _dp, _dp,
+                                     C.POINTER(_i64), C.POINTER(_i32)]),
     "femtc_global_apply_affine": (_i32, [_i32, _i32, _i32, _i32, _vp, _vp, _vp,
                                          _i64, _vp, _vp, _i64, _i32, _vp]),
     "femtc_plan_create": (_i32, [C.POINTER(_vp), _i32, _i32, _i32, _i32, _vp,

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <vector>
 
+#include "even_odd.hpp"
 #include "femtc.h"
 
 namespace
@@ -173,6 +174,21 @@ int femtc_basis_tables(int degree, int nq, double* tables)
       dphi[d * nq + q] = dv / den;
     }
   }
+  return FEMTC_OK;
+}
+
+int femtc_even_odd_tables(int degree, int nq, const double* tables,
+                          const double* weights, double* out, int64_t* size,
+                          int* symmetric)
+{
+  if (degree < 1 || degree > FEMTC_MAX_DEGREE || nq < 1 || nq > 64 ||
+      !tables || !size || !symmetric)
+    return FEMTC_ERR_INVALID_ARGUMENT;
+  const int n = degree + 1;
+  *size = femtc::eo_table_size(n, nq);
+  *symmetric = femtc::eo_symmetric(tables, n, nq) ? 1 : 0;
+  if (out)
+    femtc::eo_build(tables, weights, n, nq, out);
   return FEMTC_OK;
 }
 

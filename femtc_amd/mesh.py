@@ -48,6 +48,26 @@ def geometry_affine(J, nq: int, G=None, detJw=None, stream=None) -> None:
         _stream(stream)), "geometry_affine")
 
 
+def even_odd_tables(degree: int, nq: int, tables, weights=None):
+    """(the even-odd table set of [phi | dphi] and the weights, whether the
+    tables pass the symmetry check): what plans and solvers build."""
+    import ctypes as C
+    tab = np.ascontiguousarray(tables, dtype=np.float64)
+    w = None if weights is None else np.ascontiguousarray(weights,
+                                                          dtype=np.float64)
+    size, sym = C.c_int64(), C.c_int32()
+    wp = None if w is None else w.ctypes.data_as(_lib._dp)
+    _lib.check(_lib.lib().femtc_even_odd_tables(
+        degree, nq, tab.ctypes.data_as(_lib._dp), wp, None, C.byref(size),
+        C.byref(sym)), "even_odd_tables")
+    out = np.zeros(size.value)
+    _lib.check(_lib.lib().femtc_even_odd_tables(
+        degree, nq, tab.ctypes.data_as(_lib._dp), wp,
+        out.ctypes.data_as(_lib._dp), C.byref(size), C.byref(sym)),
+        "even_odd_tables")
+    return out, bool(sym.value)
+
+
 def basis_tables_weights(degree: int, nq: int) -> np.ndarray:
     """[phi | dphi | w]: the tables followed by the nq Gauss weights (the
     table layout of the affine-geometry operators)."""

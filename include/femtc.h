@@ -123,6 +123,21 @@ int femtc_gll_points(int degree, double* x);
 int femtc_gauss_rule(int nq, double* x, double* w);
 /* [phi | dphi] of the GLL Lagrange basis at the Gauss points, 2*N*nq values. */
 int femtc_basis_tables(int degree, int nq, double* tables);
+/* The even-odd table set that plans (femtc_plan_set_tables) and solvers
+ * build from [phi | dphi] (2*N*nq values) and the weights (nq values, may
+ * be NULL: zeros).  *size = its length in values; *symmetric = 1 when the
+ * tables pass the symmetry check (phi symmetric, dphi antisymmetric under
+ * b -> N-1-b, q -> nq-1-q, to 64 ulps of the largest entry); out (may be
+ * NULL to query the size) receives it.  Layout: four terms, phi forward
+ * (dof -> point), dphi forward, phi backward (point -> dof), dphi backward;
+ * a term with input length K, output length M and sign S (+1 phi, -1 dphi)
+ * holds block A, ceil(K/2) rows (S = +1; floor(K/2) for S = -1) of
+ * round4(ceil(M/2)) values, then block B, floor(K/2) rows (S = +1;
+ * ceil(K/2) for S = -1) of round4(floor(M/2)) values; then the weights.
+ * No counterpart in the reference (host, FP64). */
+int femtc_even_odd_tables(int degree, int nq, const double* tables,
+                          const double* weights, double* out, int64_t* size,
+                          int* symmetric);
 
 /* ---------------------------------------------------------------------- */
 /* Cell-local operators on device buffers (stream-ordered)                 */
