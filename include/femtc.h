@@ -281,7 +281,8 @@ int femtc_plan_set_tables(femtc_plan* plan, int dtype, int nq,
  * dofmap, the assembled scatter) on these buffers, reps overwriting applies
  * each (y = A x, clearing included), and keep the fastest for
  * (nq, dtype, op).  y is left cleared.  best_us (may be NULL) receives its
- * time per apply. */
+ * time per apply.  tables may be NULL as for femtc_plan_apply (the
+ * registered copy, in even-odd form where it applies, is then timed). */
 int femtc_plan_autotune(femtc_plan* plan, int op, int dtype, int nq,
                         const void* tables, const void* x, void* y,
                         const void* G, const void* detJw, int reps,
