@@ -78,6 +78,20 @@ constexpr int block_threads()
   return ((cells_per_block<Q>() * Q * Q + 63) / 64) * 64;
 }
 
+// Cells and threads per workgroup of an instance with scatter kind BW.
+template <typename T, int Q, int BW>
+constexpr int cells_per_group()
+{
+  return BW == SCATTER_ASSEMBLED ? assembled_cells(Q, (int)sizeof(T))
+                                 : cells_per_block<Q>();
+}
+
+template <typename T, int Q, int BW>
+constexpr int group_threads()
+{
+  return ((cells_per_group<T, Q, BW>() * Q * Q + 63) / 64) * 64;
+}
+
 // A geometry piece: two elements, 16 bytes for FP64 and 8 for FP32.  A
 // cell's G block (6 Q^3 elements, always a multiple of two) is 16-byte
 // aligned for FP64 and 8-byte aligned for FP32.
@@ -263,17 +277,21 @@ __device__ inline void load_geometry_half(const T* __restrict__ G,
 // LDS elements of one block: the work tensors X and Y share one area (they
 // are never live together), and a half of the geometry block aliases it at
 // the quadrature stage.
-template <typename T, int N, int Q, int OP>
+template <typename T, int N, int Q, int OP, int BW>
 constexpr int work_lds_elems()
 {
   constexpr bool AFF = OP >= OP_MASS_AFF;
   constexpr int OPB = AFF ? OP - OP_MASS_AFF : OP;
   constexpr bool GRAD = OPB != OP_MASS;
   constexpr bool GEO = GRAD && !AFF;
-  constexpr int CPB = cells_per_block<Q>();
+  constexpr int CPB = cells_per_group<T, Q, BW>();
   constexpr LdsLayout LY = lds_layout<(int)sizeof(T), N, Q>();
-  constexpr int XCELLS = CPB > 1 ? LY.XCELL : (GRAD ? 2 : 1) * Q * LY.XQ;
-  constexpr int YCELLS = CPB > 1 ? LY.YCELL : (GRAD ? 3 : 1) * N * LY.YA;
+  // per-cell strides: the tuned ones where the table has them, else the
+  // cell's own footprint
+  constexpr int XCELLS =
+      CPB > 1 && LY.XCELL ? LY.XCELL : (GRAD ? 2 : 1) * Q * LY.XQ;
+  constexpr int YCELLS =
+      CPB > 1 && LY.YCELL ? LY.YCELL : (GRAD ? 3 : 1) * N * LY.YA;
   constexpr int WCELL = XCELLS > YCELLS ? XCELLS : YCELLS;
   constexpr int WORK = CPB * WCELL;
   constexpr int GELEM = GEO ? 6 * geometry_split<Q>() * Q * Q * CPB : 0;
@@ -304,7 +322,8 @@ __device__ inline void stage_sync()
 template <typename T, int N, int Q, int OP>
 constexpr size_t blocked_work_bytes()
 {
-  return sizeof(Piece<T>) * (size_t)((work_lds_elems<T, N, Q, OP>() + 1) / 2);
+  return sizeof(Piece<T>) *
+         (size_t)((work_lds_elems<T, N, Q, OP, 1>() + 1) / 2);
 }
 
 inline size_t blocked_box(int P, int cbx, int cby, int cbz)
@@ -340,8 +359,8 @@ apply_body(const T* __restrict__ tab, const T* __restrict__ x,
   constexpr bool ASM = BW == SCATTER_ASSEMBLED;
   static_assert(!BLK || (GATHER && Q * Q <= 64), "blocked scatter layout");
   static_assert(!ASM || (GATHER && !REF), "assembled scatter");
-  constexpr int CPB = cells_per_block<Q>();
-  constexpr int BT = BLK ? 64 : block_threads<Q>(); // threads per cell group
+  constexpr int CPB = cells_per_group<T, Q, BW>();
+  constexpr int BT = BLK ? 64 : group_threads<T, Q, BW>(); // threads per group
   constexpr int NL = Q * Q;
   constexpr bool AFF = OP >= OP_MASS_AFF; // geometry from 7 values per cell
   constexpr int OPB = AFF ? OP - OP_MASS_AFF : OP;
@@ -365,8 +384,8 @@ apply_body(const T* __restrict__ tab, const T* __restrict__ x,
   constexpr int XQ = LY.XQ, XC = LY.XC, YQ = LY.YQ, YA = LY.YA;
   constexpr int SX = Q * XQ; // one X tensor
   constexpr int SY = N * YA; // one Y tensor
-  constexpr int XCELLS = CPB > 1 ? LY.XCELL : NX * SX;
-  constexpr int YCELLS = CPB > 1 ? LY.YCELL : NY * SY;
+  constexpr int XCELLS = CPB > 1 && LY.XCELL ? LY.XCELL : NX * SX;
+  constexpr int YCELLS = CPB > 1 && LY.YCELL ? LY.YCELL : NY * SY;
   static_assert(XCELLS >= NX * SX && YCELLS >= NY * SY, "layout fits");
   constexpr int NN = N * N;
   constexpr int N3 = N * N * N;
@@ -382,7 +401,7 @@ apply_body(const T* __restrict__ tab, const T* __restrict__ x,
   // LDS: X and Y share one area per cell (WCELL); a geometry half aliases
   // the whole area at the quadrature stage, when neither is live.
   constexpr int WCELL = XCELLS > YCELLS ? XCELLS : YCELLS;
-  constexpr int LDSW = (work_lds_elems<T, N, Q, OP>() + 1) / 2; // pieces
+  constexpr int LDSW = (work_lds_elems<T, N, Q, OP, BW>() + 1) / 2; // pieces
   // LDS slots
   constexpr int X_P = 0, X_G = 1;              // F1 -> F2
   constexpr int Y_PP = 0, Y_PG = 1, Y_GP = 2;  // F2 -> F3
@@ -1131,8 +1150,9 @@ constexpr int min_waves_per_simd()
 
 template <typename T, int N, int Q, int OP, bool REF, bool GATHER, int BW,
           bool EO = false>
-__global__ void __launch_bounds__(BW > 0 ? 64 * BW : block_threads<Q>(),
-                                  (min_waves_per_simd<T, Q, OP, BW>()))
+__global__ void
+__launch_bounds__((BW > 0 ? 64 * BW : group_threads<T, Q, BW>()),
+                  (min_waves_per_simd<T, Q, OP, BW>()))
     apply_cells(const T* __restrict__ tab, const T* __restrict__ x,
                 T* __restrict__ y, const T* __restrict__ G,
                 const T* __restrict__ detJw, const int32_t* __restrict__ dofmap,
@@ -1351,7 +1371,7 @@ int launch_blocked(const ApplyReq& r)
 template <typename T, int N, int Q, int OP, bool EO = false>
 int launch_assembled(const ApplyReq& r)
 {
-  constexpr int CPB = cells_per_block<Q>();
+  constexpr int CPB = cells_per_group<T, Q, SCATTER_ASSEMBLED>();
   if (r.ncx < 1 || r.ncy < 1 || r.ncz < 1 || !r.lattice ||
       (long long)r.ncx * r.ncy * r.ncz != r.num_cells)
     return FEMTC_ERR_INVALID_ARGUMENT;
@@ -1366,7 +1386,8 @@ int launch_assembled(const ApplyReq& r)
   const long long grid = 8 * ((nblk + 7) / 8);
   hipLaunchKernelGGL((apply_cells<T, N, Q, OP, false, true, SCATTER_ASSEMBLED,
                                   EO>),
-                     dim3((unsigned)grid), dim3(block_threads<Q>()), 0,
+                     dim3((unsigned)grid),
+                     dim3(group_threads<T, Q, SCATTER_ASSEMBLED>()), 0,
                      r.stream, (const T*)(EO ? r.eo_tab : r.tab),
                      (const T*)r.x, (T*)r.lattice,
                      (const T*)r.G, (const T*)r.detJw, r.dofmap,

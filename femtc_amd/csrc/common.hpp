@@ -78,6 +78,20 @@ struct ApplyReq
 // BW template value of the assembled scatter's cell-pass instances.
 constexpr int SCATTER_ASSEMBLED = -1;
 
+// Cells per workgroup of the assembled scatter's cell pass (esize: bytes
+// per value).  One wave of whole cells (Q^2 threads each) leaves lanes idle
+// at Q = 6 and 7 (36 and 49 of 64 busy); for FP32 the workgroup there takes
+// 7 and 5 cells over 4 waves (252/256, 245/256 lanes), which made the ALU-
+// bound FP32 cell pass faster.  FP64 (and the direct and blocked scatters,
+// whose atomics measured 4-19 % slower from multi-wave workgroups) keep
+// one-wave workgroups.
+constexpr int assembled_cells(int Q, int esize)
+{
+  return esize == 4 && Q == 6   ? 7
+         : esize == 4 && Q == 7 ? 5
+                                : (Q * Q <= 64 ? 64 / (Q * Q) : 1);
+}
+
 // One problem of a batched launch (device-resident descriptor).
 struct BatchDesc
 {

@@ -823,6 +823,18 @@ int time_step(femtc_solver* S, int reps, float* ms)
   return st;
 }
 
+// Energy partials one operator launch writes: one per wave of every cell
+// group (the assembled scatter's groups hold assembled_cells cells).
+int64_t energy_partials(int64_t C, int Q, bool assembled, int dtype)
+{
+  const int64_t Qq = (int64_t)Q * Q;
+  const int64_t cpg =
+      assembled ? assembled_cells(Q, dtype == FEMTC_F64 ? 8 : 4)
+                : (Qq <= 64 ? 64 / Qq : 1);
+  const int64_t waves = (cpg * Qq + 63) / 64;
+  return (C + cpg - 1) / cpg * waves;
+}
+
 // Scatter and contraction form of the operators.  The scatters allowed by
 // problem.grid/scatter (the assembled one only for the structured dofmap;
 // FEMTC_SCATTER_AUTO allows both) and the forms available (even-odd when
@@ -888,6 +900,8 @@ int choose_variant(femtc_solver* S, int structured)
   }
   S->inner_assembled = cand[best].first;
   S->even_odd = cand[best].second;
+  S->nepart = energy_partials(S->prob.num_cells, S->Q, S->inner_assembled,
+                              S->inner_dtype);
   if (!S->inner_assembled && S->lattice)
   {
     dev_free(S->lattice);
@@ -1165,16 +1179,12 @@ int femtc_solver_create(femtc_solver** out, const femtc_problem* prob,
         (st = dev_alloc(&S->rhs, vs * n)) || (st = dev_alloc(&S->e, vs * n)) ||
         (st = dev_alloc(&S->p, vs * n)) || (st = dev_alloc(&S->Ap, vs * n)))
       break;
-    {
-      // one energy partial per wave of the operator launch (the same cell
-      // blocks under either scatter)
-      const int64_t Qq = (int64_t)S->Q * S->Q;
-      const int64_t cpw = Qq <= 64 ? 64 / Qq : 1;
-      const int64_t waves = (cpw * Qq + 63) / 64;
-      S->nepart = (C + cpw - 1) / cpw * waves;
-      if ((st = dev_alloc((void**)&S->epart, 8 * S->nepart)))
-        break;
-    }
+    // one energy partial per wave of the operator launch, whose cell
+    // groups depend on the scatter (set again once it is chosen)
+    S->nepart = std::max(energy_partials(C, S->Q, false, inner_dtype),
+                         energy_partials(C, S->Q, true, inner_dtype));
+    if ((st = dev_alloc((void**)&S->epart, 8 * S->nepart)))
+      break;
     if (hipHostMalloc((void**)&S->hst, sizeof(PcgState)) != hipSuccess ||
         hipHostMalloc((void**)&S->hscal, sizeof(double)) != hipSuccess)
     {

@@ -365,8 +365,11 @@ def test_solver_even_odd_and_general_forms(inner, op, nx, ny, nz, P, Q,
             assert info["relres"] <= 1.01e-10 and res <= 1.05e-10, (key, info)
             assert rel(uh, u_ref) <= 1e-8, key
             iters[key] = info["inner_iterations"]
+    # the forms differ in rounding, and an FP32 inner solve on the atomic
+    # scatter varies by a few iterations from run to run anyway
     for scatter in (F.SCATTER_DIRECT, F.SCATTER_ASSEMBLED):
-        assert abs(iters[(True, scatter)] - iters[(False, scatter)]) <= 2, iters
+        a, b = iters[(True, scatter)], iters[(False, scatter)]
+        assert abs(a - b) <= max(2, 0.02 * max(a, b)), iters
 
 
 def test_solver_even_odd_needs_nq_equal_degree_plus_one():
