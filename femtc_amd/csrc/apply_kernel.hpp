@@ -7,7 +7,9 @@
 // LDS.  A workgroup is one wave holding CPB = 64/(Q*Q) cells (Q <= 8), or the
 // fewest waves that hold one cell (Q > 8), so the stage barriers are cheap
 // and the workgroups resident on a CU progress independently, hiding each
-// other's memory latency.
+// other's memory latency.  The FP32 cell pass of the assembled scatter at
+// Q = 6, 7 takes 4 waves of 7 and 5 cells instead (assembled_cells), for
+// busy lanes.
 //
 // Memory access.  Every stream is read with as few, as wide and as
 // contiguous wave instructions as possible (the vector-memory address path
