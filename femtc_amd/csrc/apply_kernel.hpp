@@ -1285,9 +1285,6 @@ int launch_batch_n(const BatchReq& r)
   return FEMTC_ERR_INVALID_ARGUMENT;
 }
 
-// cells per workgroup of the direct kernel, for a runtime nq
-inline int cells_per_workgroup(int nq) { return nq * nq <= 64 ? 64 / (nq * nq) : 1; }
-
 // Launch one instance: one workgroup per batch of CPB cells.
 template <typename T, int N, int Q, int OP, bool REF, bool GATHER,
           bool EO = false>
