@@ -180,7 +180,10 @@ def test_solver_assembled_scatter(inner, op, nx, ny, nz, P, Q, affine):
         assert rel(uh, u_ref) <= 1e-8, scatter
         assert rel(host(u2), uh) <= 1e-9, scatter
         iters[scatter] = info["inner_iterations"]
-    assert abs(iters[F.SCATTER_DIRECT] - iters[F.SCATTER_ASSEMBLED]) <= 2, iters
+    # an FP32 inner solve on the atomic scatter varies by a few iterations
+    # from run to run (the atomic sums' last bits)
+    a, b = iters[F.SCATTER_DIRECT], iters[F.SCATTER_ASSEMBLED]
+    assert abs(a - b) <= max(2, 0.02 * max(a, b)), iters
 
 
 @pytest.mark.parametrize("inner", [F.F64, F.F32])
