@@ -153,7 +153,7 @@ def test_solver_assembled_scatter(inner, op, nx, ny, nz, P, Q, affine):
     # the same solve with the atomic and the assembled scatter (forced, and
     # picked by timing): each meets the tolerance under the oracle's
     # operator and matches the oracle's solution; the assembled update
-    # drives CG like the atomic one (iterations +- 2)
+    # drives CG like the atomic one (iterations within 2 %)
     seed = 120 + P
     C, G, Dw, dm, ng, tab, f = _problem(op, nx, ny, nz, P, Q, seed)
     name = {F.OP_MASS: "mass", F.OP_KPM: "kpm"}[op]
