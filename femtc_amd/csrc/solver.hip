@@ -112,6 +112,34 @@ __device__ inline void combine_partials(const double* part, double (&v)[NV])
   block_sum<NV>(v);
 }
 
+// The Jacobi preconditioner D^-1 of a PCG in precision V is held as
+// Precond<V>::T: FP64 for the FP64 solve, and bfloat16 (the upper half of
+// the FP32 value, rounded to nearest even) for the FP32 inner solve of the
+// refinement.  Any fixed positive diagonal is a valid preconditioner; the
+// rounded one is within 2^-9 of D^-1 entry by entry, which moves the
+// condition number of the preconditioned operator by under 0.4 %, and the
+// outer FP64 loop fixes the accuracy of the result.  The streaming passes
+// then read 2 bytes per dof for it instead of 4.  The assembly update reads
+// the same rounded values from an FP32 copy: in its row-wise pattern 2-byte
+// loads measured slower than 4-byte ones (130 against 88 us per pass at
+// 289^3 dofs), while the streaming p-update gained (85 -> 73 us).
+template <typename V>
+struct Precond
+{
+  using T = V;
+};
+template <>
+struct Precond<float>
+{
+  using T = uint16_t;
+};
+
+__device__ inline double precond_value(double d) { return d; }
+__device__ inline float precond_value(uint16_t d)
+{
+  return __uint_as_float((uint32_t)d << 16);
+}
+
 #define GRID_LOOP(i, n)                                                      \
   for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < (n);          \
        i += (int64_t)NBLK * RB)
@@ -122,14 +150,15 @@ __device__ inline void combine_partials(const double* part, double (&v)[NV])
 template <typename V>
 __global__ void __launch_bounds__(RB)
     k_start(const V* __restrict__ rhs, V* __restrict__ e, V* __restrict__ r,
-            V* __restrict__ p, V* __restrict__ Ap, const V* __restrict__ dinv,
-            int64_t n, double* __restrict__ part)
+            V* __restrict__ p, V* __restrict__ Ap,
+            const typename Precond<V>::T* __restrict__ dinv, int64_t n,
+            double* __restrict__ part)
 {
   double s[2] = {0.0, 0.0};
   GRID_LOOP(i, n)
   {
     const V ri = rhs[i];
-    const V zi = dinv[i] * ri;
+    const V zi = (V)precond_value(dinv[i]) * ri;
     e[i] = V(0);
     r[i] = ri;
     p[i] = zi;
@@ -216,7 +245,7 @@ __global__ void __launch_bounds__(RB)
 template <typename V>
 __global__ void __launch_bounds__(RB)
     k_update(V* __restrict__ r, V* __restrict__ Ap,
-             const V* __restrict__ dinv, int64_t n,
+             const typename Precond<V>::T* __restrict__ dinv, int64_t n,
              const PcgState* __restrict__ st, double* __restrict__ part)
 {
   if (st->done)
@@ -228,7 +257,7 @@ __global__ void __launch_bounds__(RB)
     const V ri = r[i] - a * Ap[i];
     r[i] = ri;
     Ap[i] = V(0);
-    const V zi = dinv[i] * ri;
+    const V zi = (V)precond_value(dinv[i]) * ri;
     s[0] += (double)ri * (double)zi;
     s[1] += (double)ri * (double)ri;
   }
@@ -314,7 +343,7 @@ __global__ void __launch_bounds__(RB)
 template <typename V>
 __global__ void __launch_bounds__(RB)
     k_pupdate(V* __restrict__ p, V* __restrict__ e, const V* __restrict__ r,
-              const V* __restrict__ dinv, int64_t n,
+              const typename Precond<V>::T* __restrict__ dinv, int64_t n,
               const PcgState* __restrict__ st)
 {
   if (!st->epend)
@@ -330,7 +359,7 @@ __global__ void __launch_bounds__(RB)
   {
     const V pi = p[i];
     e[i] += a * pi;
-    p[i] = dinv[i] * r[i] + b * pi;
+    p[i] = (V)precond_value(dinv[i]) * r[i] + b * pi;
   }
 }
 
@@ -391,6 +420,22 @@ __global__ void __launch_bounds__(RB)
     k_convert(const A* __restrict__ x, B* __restrict__ y, int64_t n)
 {
   GRID_LOOP(i, n) { y[i] = (B)x[i]; }
+}
+
+// D^-1 for the FP32 inner solve: FP64 -> FP32 -> bfloat16, rounded to
+// nearest even (the values are finite: k_invert writes 1/d or 0), and the
+// same rounded values as FP32
+__global__ void __launch_bounds__(RB)
+    k_precond_bf16(const double* __restrict__ x, uint16_t* __restrict__ y,
+                   float* __restrict__ yf, int64_t n)
+{
+  GRID_LOOP(i, n)
+  {
+    const uint32_t u = __float_as_uint((float)x[i]);
+    const uint32_t h = (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+    y[i] = (uint16_t)h;
+    yf[i] = __uint_as_float(h << 16);
+  }
 }
 
 __global__ void __launch_bounds__(RB)
@@ -580,8 +625,9 @@ struct femtc_solver
   void* r = nullptr;
   void* p = nullptr;
   void* Ap = nullptr;
-  void* dinv = nullptr;
-  void* tab = nullptr;   // inner-precision tables and geometry
+  void* dinv = nullptr;   // D^-1 as Precond<V>::T
+  void* dinv_v = nullptr; // the same values as V (the assembly pass)
+  void* tab = nullptr;    // inner-precision tables and geometry
   void* G = nullptr;     // per point, or per cell (inner_affine)
   void* detJw = nullptr;
   // affine cells (problem.J given): the inner operator computes its
@@ -663,6 +709,7 @@ void destroy(femtc_solver* S)
   {
     dev_free(S->r);
     dev_free(S->dinv);
+    dev_free(S->dinv_v);
     dev_free(S->tab);
     dev_free(S->G);
     dev_free(S->detJw);
@@ -757,7 +804,7 @@ int launch_update(femtc_solver* S, hipStream_t s)
   {
     const V* W = (const V*)S->lattice;
     V* r = (V*)S->r;
-    const V* dinv = (const V*)S->dinv;
+    const V* dinv = (const V*)S->dinv_v;
     const LatticeGrid L = lattice_grid(S);
     const PcgState* st = S->st;
     switch (L.P)
@@ -782,7 +829,8 @@ int launch_update(femtc_solver* S, hipStream_t s)
   }
   else
     hipLaunchKernelGGL(k_update<V>, dim3(NBLK), dim3(RB), 0, s, (V*)S->r,
-                       (V*)S->Ap, (const V*)S->dinv, S->n,
+                       (V*)S->Ap, (const typename Precond<V>::T*)S->dinv,
+                       S->n,
                        (const PcgState*)S->st, S->part);
   return hip_status(hipGetLastError());
 }
@@ -955,7 +1003,8 @@ int enqueue_batch(femtc_solver* S, hipStream_t s)
     hipLaunchKernelGGL(k_beta, dim3(1), dim3(RB), 0, s,
                        (const double*)S->part, S->st);
     hipLaunchKernelGGL(k_pupdate<V>, dim3(NBLK), dim3(RB), 0, s, (V*)S->p,
-                       (V*)S->e, (const V*)S->r, (const V*)S->dinv, S->n,
+                       (V*)S->e, (const V*)S->r,
+                       (const typename Precond<V>::T*)S->dinv, S->n,
                        (const PcgState*)S->st);
   }
   return hip_status(hipGetLastError());
@@ -1014,7 +1063,7 @@ int pcg_run(femtc_solver* S, double rtol, int maxit, int* iters, int* status)
   const int64_t n = S->n;
   hipLaunchKernelGGL(k_start<V>, dim3(NBLK), dim3(RB), 0, s, (const V*)S->rhs,
                      (V*)S->e, (V*)S->r, (V*)S->p, (V*)S->Ap,
-                     (const V*)S->dinv, n, S->part);
+                     (const typename Precond<V>::T*)S->dinv, n, S->part);
   hipLaunchKernelGGL(k_reduce_start, dim3(1), dim3(RB), 0, s,
                      (const double*)S->part, S->st, rtol, maxit);
   LAUNCH_CHECK();
@@ -1233,6 +1282,7 @@ int femtc_solver_create(femtc_solver** out, const femtc_problem* prob,
     {
       S->r = S->r64;
       S->dinv = S->dinv64;
+      S->dinv_v = S->dinv64;
       S->tab = aff ? (void*)S->tabw64 : (void*)prob->tables;
       S->G = aff ? (void*)S->cg64 : (void*)prob->G;
       S->detJw = aff ? nullptr : (void*)prob->detJw;
@@ -1240,12 +1290,14 @@ int femtc_solver_create(femtc_solver** out, const femtc_problem* prob,
     else if (aff)
     {
       // FP32 inner operator on affine cells: 7 values per cell
-      if ((st = dev_alloc(&S->r, 4 * n)) || (st = dev_alloc(&S->dinv, 4 * n)) ||
+      if ((st = dev_alloc(&S->r, 4 * n)) || (st = dev_alloc(&S->dinv, 2 * n)) ||
+          (st = dev_alloc(&S->dinv_v, 4 * n)) ||
           (st = dev_alloc(&S->tab, 4 * (ntab + S->Q))) ||
           (st = dev_alloc(&S->G, 4 * CELLGEO * C)))
         break;
-      hipLaunchKernelGGL((k_convert<double, float>), dim3(NBLK), dim3(RB), 0,
-                         s, (const double*)S->dinv64, (float*)S->dinv, n);
+      hipLaunchKernelGGL(k_precond_bf16, dim3(NBLK), dim3(RB), 0, s,
+                         (const double*)S->dinv64, (uint16_t*)S->dinv,
+                         (float*)S->dinv_v, n);
       hipLaunchKernelGGL((k_convert<double, float>), dim3(NBLK), dim3(RB), 0,
                          s, (const double*)S->tabw64, (float*)S->tab,
                          ntab + S->Q);
@@ -1255,11 +1307,13 @@ int femtc_solver_create(femtc_solver** out, const femtc_problem* prob,
     }
     else
     {
-      if ((st = dev_alloc(&S->r, 4 * n)) || (st = dev_alloc(&S->dinv, 4 * n)) ||
+      if ((st = dev_alloc(&S->r, 4 * n)) || (st = dev_alloc(&S->dinv, 2 * n)) ||
+          (st = dev_alloc(&S->dinv_v, 4 * n)) ||
           (st = dev_alloc(&S->tab, 4 * 2 * S->N * S->Q)))
         break;
-      hipLaunchKernelGGL((k_convert<double, float>), dim3(NBLK), dim3(RB), 0,
-                         s, (const double*)S->dinv64, (float*)S->dinv, n);
+      hipLaunchKernelGGL(k_precond_bf16, dim3(NBLK), dim3(RB), 0, s,
+                         (const double*)S->dinv64, (uint16_t*)S->dinv,
+                         (float*)S->dinv_v, n);
       hipLaunchKernelGGL((k_convert<double, float>), dim3(NBLK), dim3(RB), 0,
                          s, prob->tables, (float*)S->tab,
                          (int64_t)2 * S->N * S->Q);
