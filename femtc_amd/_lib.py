@@ -38,6 +38,7 @@ SIGNATURES = {
     "femtc_last_error": (C.c_char_p, []),
     "femtc_version": (_i32, []),
     "femtc_max_degree": (_i32, []),
+    "femtc_max_fused_degree": (_i32, []),
     "femtc_string2order": (_i32, [C.c_char_p, C.POINTER(_i32)]),
     "femtc_check_ladder": (_i32, [_i32, _i32, _i32, _i32]),
     "femtc_device_count": (_i32, [C.POINTER(_i32)]),

@@ -55,6 +55,16 @@ int check_shape(int degree, int nq, int mode)
   return FEMTC_OK;
 }
 
+int check_shape_fused(int degree, int nq, int mode)
+{
+  if (degree > FEMTC_MAX_FUSED_DEGREE)
+    return fail(FEMTC_ERR_DEGREE_NOT_SUPPORTED,
+                "degree not supported: plans, solvers and batches cover "
+                "degrees 1.." +
+                    std::to_string(FEMTC_MAX_FUSED_DEGREE));
+  return check_shape(degree, nq, mode);
+}
+
 int check_dtype(int dtype)
 {
   if (dtype != FEMTC_F64 && dtype != FEMTC_F32)
@@ -73,6 +83,8 @@ int run(const ApplyReq& r)
                 "G must be aligned to two elements (16 bytes for FP64, 8 for "
                 "FP32)");
   ApplyLauncher f = launcher_for(r.N);
+  if (!f && r.N <= FEMTC_MAX_DEGREE + 1)
+    f = launch_apply_general; // degrees above the fused kernels
   if (!f)
     return fail(FEMTC_ERR_DEGREE_NOT_SUPPORTED, "degree not supported");
   const int st = f(r);
@@ -138,6 +150,8 @@ const char* femtc_last_error(void) { return g_last_error.c_str(); }
 int femtc_version(void) { return FEMTC_VERSION; }
 
 int femtc_max_degree(void) { return FEMTC_MAX_DEGREE; }
+
+int femtc_max_fused_degree(void) { return FEMTC_MAX_FUSED_DEGREE; }
 
 int femtc_string2order(const char* name, int* order)
 {

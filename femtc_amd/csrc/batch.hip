@@ -215,7 +215,7 @@ void release(femtc_batch* b)
 int check_item(const femtc_batch_item& it, int i)
 {
   const std::string where = "batch item " + std::to_string(i) + ": ";
-  int st = check_shape(it.degree, it.nq, FEMTC_MODE_CORRECT);
+  int st = check_shape_fused(it.degree, it.nq, FEMTC_MODE_CORRECT);
   if (st != FEMTC_OK)
     return st;
   if ((st = check_dtype(it.dtype)) != FEMTC_OK)

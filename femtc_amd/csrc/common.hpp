@@ -156,6 +156,8 @@ int launch_apply_n6(const ApplyReq&);
 int launch_apply_n7(const ApplyReq&);
 int launch_apply_n8(const ApplyReq&);
 int launch_apply_n9(const ApplyReq&);
+// N = 10..26 (apply_general.hip); also accepts smaller N
+int launch_apply_general(const ApplyReq&);
 
 inline int hip_status(hipError_t e)
 {
@@ -165,6 +167,8 @@ inline int hip_status(hipError_t e)
 // Shared validation and dispatch (api.cpp).
 int fail(int status, const std::string& msg);
 int check_shape(int degree, int nq, int mode);
+// check_shape, and a degree the fused kernels cover (plans, solvers, batches)
+int check_shape_fused(int degree, int nq, int mode);
 int check_dtype(int dtype);
 int run(const ApplyReq& r); // validated request -> per-degree launcher
 int fill_zero(void* y, int64_t n, int dtype, hipStream_t s);

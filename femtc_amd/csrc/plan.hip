@@ -150,7 +150,7 @@ int verify_structured(const int32_t* dofmap, int nx, int ny, int nz,
 {
   *ok = 0;
   if (!dofmap || nx < 1 || ny < 1 || nz < 1 || degree < 1 ||
-      degree > FEMTC_MAX_DEGREE)
+      degree > FEMTC_MAX_FUSED_DEGREE)
     return FEMTC_ERR_INVALID_ARGUMENT;
   const int P = degree;
   if (ndofs != ((int64_t)nx * P + 1) * ((int64_t)ny * P + 1) *
@@ -357,7 +357,7 @@ int check_apply_args(const femtc_plan* pl, int op, int dtype, int nq,
 {
   if (!pl)
     return fail(FEMTC_ERR_INVALID_ARGUMENT, "plan is NULL");
-  int st = check_shape(pl->degree, nq, FEMTC_MODE_CORRECT);
+  int st = check_shape_fused(pl->degree, nq, FEMTC_MODE_CORRECT);
   if (st != FEMTC_OK)
     return st;
   if ((st = check_dtype(dtype)) != FEMTC_OK)
@@ -429,7 +429,7 @@ int femtc_plan_create(femtc_plan** out, int nx, int ny, int nz, int degree,
   if (nx < 1 || ny < 1 || nz < 1 || ndofs < 1 || !dofmap)
     return fail(FEMTC_ERR_INVALID_ARGUMENT,
                 "plan needs nx, ny, nz >= 1, ndofs >= 1 and a dofmap");
-  if (degree < 1 || degree > FEMTC_MAX_DEGREE)
+  if (degree < 1 || degree > FEMTC_MAX_FUSED_DEGREE)
     return fail(FEMTC_ERR_DEGREE_NOT_SUPPORTED, "degree not supported");
   if (ndofs > 0x7fffffffLL)
     return fail(FEMTC_ERR_INVALID_ARGUMENT, "ndofs exceeds int32 indexing");
@@ -580,7 +580,7 @@ int femtc_plan_set_tables(femtc_plan* plan, int dtype, int nq,
     *even_odd = 0;
   if (!plan || !tables)
     return fail(FEMTC_ERR_INVALID_ARGUMENT, "plan and tables are required");
-  int st = check_shape(plan->degree, nq, FEMTC_MODE_CORRECT);
+  int st = check_shape_fused(plan->degree, nq, FEMTC_MODE_CORRECT);
   if (st == FEMTC_OK)
     st = check_dtype(dtype);
   if (st != FEMTC_OK)

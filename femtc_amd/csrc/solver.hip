@@ -1127,7 +1127,7 @@ int check_problem(const femtc_problem* P)
     return fail(FEMTC_ERR_INVALID_ARGUMENT, "problem is NULL");
   if (P->op < FEMTC_OP_MASS || P->op > FEMTC_OP_KPM)
     return fail(FEMTC_ERR_INVALID_ARGUMENT, "op must be a femtc_op");
-  int st = check_shape(P->degree, P->nq, FEMTC_MODE_CORRECT);
+  int st = check_shape_fused(P->degree, P->nq, FEMTC_MODE_CORRECT);
   if (st != FEMTC_OK)
     return st;
   if (P->num_cells < 0 || P->ndofs < 0)
@@ -1168,7 +1168,7 @@ int femtc_global_diagonal(int op, int dtype, int degree, int nq,
                           const int32_t* dofmap, const void* G,
                           const void* detJw, int64_t num_cells, void* stream)
 {
-  int st = check_shape(degree, nq, FEMTC_MODE_CORRECT);
+  int st = check_shape_fused(degree, nq, FEMTC_MODE_CORRECT);
   if (st != FEMTC_OK)
     return st;
   if ((st = check_dtype(dtype)) != FEMTC_OK)

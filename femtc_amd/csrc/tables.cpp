@@ -181,7 +181,7 @@ int femtc_even_odd_tables(int degree, int nq, const double* tables,
                           const double* weights, double* out, int64_t* size,
                           int* symmetric)
 {
-  if (degree < 1 || degree > FEMTC_MAX_DEGREE || nq < 1 || nq > 64 ||
+  if (degree < 1 || degree > FEMTC_MAX_FUSED_DEGREE || nq < 1 || nq > 64 ||
       !tables || !size || !symmetric)
     return FEMTC_ERR_INVALID_ARGUMENT;
   const int n = degree + 1;

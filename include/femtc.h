@@ -31,7 +31,11 @@ extern "C" {
 #endif
 
 #define FEMTC_VERSION 1
-#define FEMTC_MAX_DEGREE 8
+/* Operator apply (cell-local and through a dofmap) and the reference
+ * ladder: degrees 1..25 (src/mass.hpp:52-138).  Plans, the solver, batches
+ * and the even-odd tables run degrees 1..FEMTC_MAX_FUSED_DEGREE. */
+#define FEMTC_MAX_DEGREE 25
+#define FEMTC_MAX_FUSED_DEGREE 8
 
 typedef enum femtc_status
 {
@@ -92,9 +96,12 @@ typedef enum femtc_op
 const char* femtc_status_string(int status);
 const char* femtc_last_error(void);
 int femtc_version(void);
-/* Highest supported degree for the given quadrature offset (nq - degree - 1
- * in {0,1,2}); degrees 1..FEMTC_MAX_DEGREE are supported. */
+/* Highest degree of the operator apply entry points (FEMTC_MAX_DEGREE), for
+ * every quadrature offset nq - degree - 1 in {0,1,2}. */
 int femtc_max_degree(void);
+/* Highest degree of plans, solvers, batches, the server's solves and the
+ * even-odd tables (FEMTC_MAX_FUSED_DEGREE). */
+int femtc_max_fused_degree(void);
 
 /* linalg::string2order (src/linalg.hpp:21-37): "ijk".."kji" -> *order;
  * anything else returns FEMTC_ERR_INVALID_ORDER_STRING. */

@@ -42,7 +42,8 @@ def test_library_exports_every_declared_symbol():
 def test_status_strings_and_version():
     L = F.lib()
     assert L.femtc_version() == 1
-    assert L.femtc_max_degree() == 8
+    assert L.femtc_max_degree() == 25
+    assert L.femtc_max_fused_degree() == 8
     assert _lib.status_string(_lib.ERR_DEGREE_NOT_SUPPORTED) == "degree not supported"
     assert _lib.status_string(_lib.ERR_BLOCK_NOT_SUPPORTED) == "Block size not supported"
     assert _lib.status_string(_lib.ERR_INVALID_ORDER_STRING) == "Invalid loop order"
@@ -66,8 +67,11 @@ def test_ladder_validation_matches_reference_messages():
         assert _lib.status_string(st) == str(msg)
     assert L.femtc_check_ladder(3, 4, 8, 16) == 0
     assert L.femtc_check_ladder(3, 6, 0, 0) == _lib.ERR_ORDER_NOT_SUPPORTED
-    # degrees the reference instantiates but the GPU build does not
-    assert L.femtc_check_ladder(9, 0, 0, 0) == _lib.ERR_DEGREE_NOT_SUPPORTED
+    # the reference instantiates degrees 1..25 (src/mass.hpp:52-138)
+    for deg in (9, 17, 25):
+        assert L.femtc_check_ladder(deg, 0, 0, 0) == 0
+    assert L.femtc_check_ladder(26, 0, 0, 0) == _lib.ERR_DEGREE_NOT_SUPPORTED
+    assert L.femtc_check_ladder(0, 0, 0, 0) == _lib.ERR_DEGREE_NOT_SUPPORTED
 
 
 @pytest.mark.parametrize("P", [1, 2, 3, 4, 5, 6, 7, 8])
@@ -82,11 +86,30 @@ def test_tables_match_oracle(P):
                                    rtol=1e-13, atol=1e-13)
 
 
+@pytest.mark.parametrize("P", [9, 12, 17, 25])
+def test_tables_match_oracle_high_degree(P):
+    # the general-degree kernels' tables: GLL nodes from two independent
+    # root finders, Lagrange values and derivatives at Gauss points
+    np.testing.assert_allclose(F.gll_points(P), O.gll_points(P), atol=1e-14)
+    for Q in (P + 1, P + 3):
+        t, o = F.basis_tables(P, Q), O.basis_tables(P, Q)
+        N = P + 1
+        # partition of unity and zero-sum derivatives (exact for any nodes)
+        np.testing.assert_allclose(t[:N * Q].reshape(N, Q).sum(0), 1.0,
+                                   atol=1e-12)
+        np.testing.assert_allclose(t[N * Q:].reshape(N, Q).sum(0), 0.0,
+                                   atol=1e-9)
+        scale = np.abs(o).max()
+        assert np.abs(t - o).max() <= 1e-11 * scale
+
+
 def test_argument_errors_without_device_work():
     L = F.lib()
     # unsupported degree / shape are rejected before any HIP call
-    assert L.femtc_mass_apply(0, 9, 10, 0, None, None, None, None, 10, None) \
+    assert L.femtc_mass_apply(0, 26, 27, 0, None, None, None, None, 10, None) \
         == _lib.ERR_DEGREE_NOT_SUPPORTED
+    assert L.femtc_mass_apply(0, 25, 26, 0, None, None, None, None, 10, None) \
+        == _lib.ERR_INVALID_ARGUMENT  # degree 25 runs; the buffers are null
     assert L.femtc_mass_apply(0, 3, 3, 0, None, None, None, None, 10, None) \
         == _lib.ERR_INVALID_ARGUMENT
     assert L.femtc_stiffness_apply(0, 3, 5, 1, None, None, None, None, 10, None) \

@@ -82,3 +82,29 @@ def test_gemm_blocked_live_reference_large_degrees():
             assert L.ref_gemm_blocked(P, order, 8, 16, a, b, ref) == 0
             c = O.gemm_blocked(N, N, N * N, order, 8, 16, a, b)
             np.testing.assert_array_equal(c, ref)
+
+
+@pytest.mark.skipif(not (O.ref_available("mass") and
+                         O.ref_available("stiffness")),
+                    reason="compiled reference not present")
+@pytest.mark.parametrize("P", [5, 9, 16, 25])
+def test_literal_operators_live_reference_high_degrees(P):
+    # degrees past the fixture, up to the top of the reference's ladder
+    # (src/mass.hpp:52-138, src/stiffness.hpp:102-188), bit for bit
+    N = P + 1
+    ncell = 2
+    phi2 = O.uniform(600 + P, 2 * N * N, 0, 1)
+    U = O.uniform(601 + P, ncell * N ** 3, 0, 1)
+    D = O.uniform(602 + P, ncell * N ** 3, 0, 1)
+    G = O.uniform(603 + P, ncell * N ** 3 * 6, 0, 1)
+    ref = np.zeros(ncell * N ** 3)
+    O.ref_lib("mass").ref_mass_operator_f64(phi2[:N * N].copy(), U.copy(),
+                                            ref, D.copy(), ncell, P, 0, 0, 0)
+    np.testing.assert_array_equal(O.mass_literal(P, phi2[:N * N], U, D, ncell),
+                                  ref)
+    ref = np.zeros(ncell * N ** 3)
+    O.ref_lib("stiffness").ref_stiffness_operator_f64(phi2.copy(), U.copy(),
+                                                      ref, G.copy(), ncell, P,
+                                                      0, 0, 0)
+    np.testing.assert_array_equal(O.stiffness_literal(P, phi2, U, G, ncell),
+                                  ref)
