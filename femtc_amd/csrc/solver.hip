@@ -119,10 +119,10 @@ __device__ inline void combine_partials(const double* part, double (&v)[NV])
 // rounded one is within 2^-9 of D^-1 entry by entry, which moves the
 // condition number of the preconditioned operator by under 0.4 %, and the
 // outer FP64 loop fixes the accuracy of the result.  The streaming passes
-// then read 2 bytes per dof for it instead of 4.  The assembly update reads
-// the same rounded values from an FP32 copy: in its row-wise pattern 2-byte
-// loads measured slower than 4-byte ones (130 against 88 us per pass at
-// 289^3 dofs), while the streaming p-update gained (85 -> 73 us).
+// then read 2 bytes per dof for it instead of 4 (the p-update at 289^3
+// dofs: 95 -> 84 us).  The assembly update reads the same rounded values
+// from an FP32 copy: in its row-wise pattern the 2-byte loads made the pass
+// take 1.5 times as long.
 template <typename V>
 struct Precond
 {
