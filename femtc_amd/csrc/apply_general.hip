@@ -38,8 +38,12 @@ namespace
 
 constexpr int GT = 256;       // threads per workgroup
 constexpr int GEN_NMAX = 26;  // dofs per direction
-constexpr int GEN_KMAX = 3;   // output columns per thread: ceil(26^2 / 256)
-static_assert(GEN_KMAX * GT >= GEN_NMAX * GEN_NMAX, "columns per thread");
+// Register shapes of the per-thread output columns, KMAX columns of up to
+// NMAX values: (1, 16) for N <= 16, (2, 22) for N <= 22, (3, 26) above, so
+// lower degrees keep more waves resident.
+static_assert(3 * GT >= GEN_NMAX * GEN_NMAX && 2 * GT >= 22 * 22 &&
+                  GT >= 16 * 16,
+              "columns per thread");
 
 __device__ inline void gen_atomic_add(double* p, double v)
 {
@@ -58,7 +62,7 @@ inline size_t gen_lds_elems(int N, int Q, bool ref, bool aff)
   return (ref ? 4 : 2) * NQ + (aff ? Q : 0) + X + 3 * NQ;
 }
 
-template <typename T, int OP, bool REF, bool GATHER>
+template <typename T, int OP, bool REF, bool GATHER, int KMAX, int NMAX>
 __global__ void __launch_bounds__(GT)
     gen_cells(const T* __restrict__ tab, const T* __restrict__ x,
               T* __restrict__ y, const T* __restrict__ G,
@@ -129,11 +133,11 @@ __global__ void __launch_bounds__(GT)
 
   for (int64_t cell = blockIdx.x; cell < C; cell += gridDim.x)
   {
-    T w[GEN_KMAX][GEN_NMAX];
+    T w[KMAX][NMAX];
 #pragma unroll
-    for (int k = 0; k < GEN_KMAX; ++k)
+    for (int k = 0; k < KMAX; ++k)
 #pragma unroll
-      for (int a = 0; a < GEN_NMAX; ++a)
+      for (int a = 0; a < NMAX; ++a)
         w[k][a] = T(0);
     const int64_t cb = cell * (int64_t)N3;
     T cg[AFF ? CELLGEO : 1];
@@ -148,7 +152,7 @@ __global__ void __launch_bounds__(GT)
     {
       // F_a: thread per column, U streamed from global (L2-resident)
 #pragma unroll
-      for (int k = 0; k < GEN_KMAX; ++k)
+      for (int k = 0; k < KMAX; ++k)
       {
         const int col = tid + k * GT;
         if (col < N2)
@@ -277,7 +281,7 @@ __global__ void __launch_bounds__(GT)
       // plane's F_a writes X only (last read in B_c, before the barrier
       // above) and its F_b writes Y after the barrier that follows F_a.
 #pragma unroll
-      for (int k = 0; k < GEN_KMAX; ++k)
+      for (int k = 0; k < KMAX; ++k)
       {
         const int col = tid + k * GT;
         if (col < N2)
@@ -296,7 +300,7 @@ __global__ void __launch_bounds__(GT)
             e2 += p * D3[di];
           }
 #pragma unroll
-          for (int a = 0; a < GEN_NMAX; ++a)
+          for (int a = 0; a < NMAX; ++a)
           {
             if (a < N)
             {
@@ -311,13 +315,13 @@ __global__ void __launch_bounds__(GT)
     }
 
 #pragma unroll
-    for (int k = 0; k < GEN_KMAX; ++k)
+    for (int k = 0; k < KMAX; ++k)
     {
       const int col = tid + k * GT;
       if (col < N2)
       {
 #pragma unroll
-        for (int a = 0; a < GEN_NMAX; ++a)
+        for (int a = 0; a < NMAX; ++a)
         {
           if (a < N)
           {
@@ -334,19 +338,31 @@ __global__ void __launch_bounds__(GT)
   }
 }
 
-template <typename T, int OP, bool REF, bool GATHER>
-int gen_launch(const ApplyReq& r)
+template <typename T, int OP, bool REF, bool GATHER, int KMAX, int NMAX>
+int gen_launch_shape(const ApplyReq& r)
 {
   constexpr bool AFF = OP >= OP_MASS_AFF;
+  if (r.N > NMAX || r.N * r.N > KMAX * GT)
+    return FEMTC_ERR_INVALID_ARGUMENT;
   const size_t bytes = gen_lds_elems(r.N, r.Q, REF, AFF) * sizeof(T);
   if (bytes > 64 * 1024)
     return FEMTC_ERR_INVALID_ARGUMENT;
   const int64_t grid64 = std::min<int64_t>(r.num_cells, 8192);
-  hipLaunchKernelGGL((gen_cells<T, OP, REF, GATHER>), dim3((unsigned)grid64),
-                     dim3(GT), bytes, r.stream, (const T*)r.tab,
-                     (const T*)r.x, (T*)r.y, (const T*)r.G,
+  hipLaunchKernelGGL((gen_cells<T, OP, REF, GATHER, KMAX, NMAX>),
+                     dim3((unsigned)grid64), dim3(GT), bytes, r.stream,
+                     (const T*)r.tab, (const T*)r.x, (T*)r.y, (const T*)r.G,
                      (const T*)r.detJw, r.dofmap, r.num_cells, r.N, r.Q);
   return hipGetLastError() == hipSuccess ? FEMTC_OK : FEMTC_ERR_HIP;
+}
+
+template <typename T, int OP, bool REF, bool GATHER>
+int gen_launch(const ApplyReq& r)
+{
+  if (r.N <= 16)
+    return gen_launch_shape<T, OP, REF, GATHER, 1, 16>(r);
+  if (r.N <= 22)
+    return gen_launch_shape<T, OP, REF, GATHER, 2, 22>(r);
+  return gen_launch_shape<T, OP, REF, GATHER, 3, GEN_NMAX>(r);
 }
 
 template <typename T, int OP, bool REF>

@@ -59,7 +59,8 @@ def _apply(op, dt, tab, U, W, G, Dw, ncell, P, Q, mode=F.MODE_CORRECT):
                     ncell, P, Q)
 
 
-CELL_SHAPES = [(P, Q) for P in (9, 10, 13, 17, 25) for Q in (P + 1, P + 3)]
+CELL_SHAPES = [(P, Q) for P in (9, 10, 13, 15, 17, 21, 25)
+               for Q in (P + 1, P + 3)]
 
 
 @pytest.mark.parametrize("P,Q", CELL_SHAPES)
