@@ -4,7 +4,7 @@
 // or gathered through a dofmap with an atomic scatter.
 //
 // The fused kernels hold a whole cell per wave in registers and LDS, which
-// stops at N = 9.  Here one workgroup of GT threads takes one cell at a time
+// stops at N = 9.  Here one workgroup (256 to 768 threads) takes one cell at a time
 // and walks it one quadrature plane qa at a time, so LDS holds only a few
 // planes (at most 60 KB at N = 26, Q = 28 in FP64) and each thread keeps
 // the output column it owns in registers:
@@ -36,14 +36,13 @@ namespace femtc
 namespace
 {
 
-constexpr int GT = 256;       // threads per workgroup
-constexpr int GEN_NMAX = 26;  // dofs per direction
-// Register shapes of the per-thread output columns, KMAX columns of up to
-// NMAX values: (1, 16) for N <= 16, (2, 22) for N <= 22, (3, 26) above, so
-// lower degrees keep more waves resident.
-static_assert(3 * GT >= GEN_NMAX * GEN_NMAX && 2 * GT >= 22 * 22 &&
-                  GT >= 16 * 16,
-              "columns per thread");
+constexpr int GEN_NMAX = 26; // dofs per direction
+// Workgroup shapes: BT threads, each owning one output column of up to NMAX
+// values in registers: (256, 16) for N <= 16, (512, 22) for N <= 22,
+// (768, 26) above.  One column per thread keeps the accumulator small, so
+// more waves stay resident.
+static_assert(256 >= 16 * 16 && 512 >= 22 * 22 && 768 >= GEN_NMAX * GEN_NMAX,
+              "one column per thread");
 
 __device__ inline void gen_atomic_add(double* p, double v)
 {
@@ -62,7 +61,8 @@ inline size_t gen_lds_elems(int N, int Q, bool ref, bool aff)
   return (ref ? 4 : 2) * NQ + (aff ? Q : 0) + X + 3 * NQ;
 }
 
-template <typename T, int OP, bool REF, bool GATHER, int KMAX, int NMAX>
+template <typename T, int OP, bool REF, bool GATHER, int KMAX, int NMAX,
+          int GT>
 __global__ void __launch_bounds__(GT)
     gen_cells(const T* __restrict__ tab, const T* __restrict__ x,
               T* __restrict__ y, const T* __restrict__ G,
@@ -338,7 +338,8 @@ __global__ void __launch_bounds__(GT)
   }
 }
 
-template <typename T, int OP, bool REF, bool GATHER, int KMAX, int NMAX>
+template <typename T, int OP, bool REF, bool GATHER, int KMAX, int NMAX,
+          int GT>
 int gen_launch_shape(const ApplyReq& r)
 {
   constexpr bool AFF = OP >= OP_MASS_AFF;
@@ -348,7 +349,7 @@ int gen_launch_shape(const ApplyReq& r)
   if (bytes > 64 * 1024)
     return FEMTC_ERR_INVALID_ARGUMENT;
   const int64_t grid64 = std::min<int64_t>(r.num_cells, 8192);
-  hipLaunchKernelGGL((gen_cells<T, OP, REF, GATHER, KMAX, NMAX>),
+  hipLaunchKernelGGL((gen_cells<T, OP, REF, GATHER, KMAX, NMAX, GT>),
                      dim3((unsigned)grid64), dim3(GT), bytes, r.stream,
                      (const T*)r.tab, (const T*)r.x, (T*)r.y, (const T*)r.G,
                      (const T*)r.detJw, r.dofmap, r.num_cells, r.N, r.Q);
@@ -359,10 +360,10 @@ template <typename T, int OP, bool REF, bool GATHER>
 int gen_launch(const ApplyReq& r)
 {
   if (r.N <= 16)
-    return gen_launch_shape<T, OP, REF, GATHER, 1, 16>(r);
+    return gen_launch_shape<T, OP, REF, GATHER, 1, 16, 256>(r);
   if (r.N <= 22)
-    return gen_launch_shape<T, OP, REF, GATHER, 2, 22>(r);
-  return gen_launch_shape<T, OP, REF, GATHER, 3, GEN_NMAX>(r);
+    return gen_launch_shape<T, OP, REF, GATHER, 1, 22, 512>(r);
+  return gen_launch_shape<T, OP, REF, GATHER, 1, GEN_NMAX, 768>(r);
 }
 
 template <typename T, int OP, bool REF>
