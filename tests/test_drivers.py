@@ -55,7 +55,10 @@ def _fields(line):
     ("bench_mass", (2097152, 3, "ijk", 0, 0, "double")),
     ("bench_mass", (1000000, 5, "kij", 8, 16, "float", "correct")),
     ("bench_stiffness", (1000000, 4, "kij", 8, 16, "double")),
-    ("bench_stiffness", (500000, 6, "ikj", 4, 4, "double", "correct"))])
+    ("bench_stiffness", (500000, 6, "ikj", 4, 4, "double", "correct")),
+    # degrees past the fused kernels, up to the top of the reference's ladder
+    ("bench_mass", (1000000, 12, "ijk", 0, 0, "double")),
+    ("bench_stiffness", (1000000, 25, "kji", 16, 8, "double"))])
 def test_driver_output_line_on_gpu(name, args):
     r = _run(name, *args)
     assert r.returncode == 0, r.stderr
