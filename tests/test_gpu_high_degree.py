@@ -136,7 +136,7 @@ def _box(nx, ny, nz, P, Q, seed):
 
 
 @pytest.mark.parametrize("nx,ny,nz,P,Q", [(2, 2, 2, 9, 10), (2, 1, 2, 11, 13),
-                                          (1, 2, 1, 16, 17)])
+                                          (1, 2, 1, 16, 17), (1, 1, 2, 24, 25)])
 @pytest.mark.parametrize("op", [F.OP_MASS, F.OP_STIFFNESS, F.OP_KPM])
 def test_global_operator_vs_oracle(op, nx, ny, nz, P, Q):
     C, J, G, Dw, dm, ng = _box(nx, ny, nz, P, Q, 60 + P)
@@ -156,7 +156,7 @@ def test_global_operator_vs_oracle(op, nx, ny, nz, P, Q):
     assert rel(host(y), ref + y0) <= TOL64
 
 
-@pytest.mark.parametrize("P,Q", [(9, 10), (12, 14)])
+@pytest.mark.parametrize("P,Q", [(9, 10), (12, 14), (17, 18), (23, 25)])
 @pytest.mark.parametrize("op", [F.OP_MASS, F.OP_STIFFNESS, F.OP_KPM])
 def test_affine_operator_vs_oracle(op, P, Q):
     nx, ny, nz = 2, 1, 2
