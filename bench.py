@@ -311,6 +311,55 @@ def bench_apply_p4(torch, F, steps=30, warmup=60):
                      "roofline_frac": ab / (pms * 1e-3) / 1e9 / HBM_PEAK_GBS}}
 
 
+def bench_high_degree(torch, F, reps=10):
+    """Cell-local mass and stiffness applies at degrees 9..25 (the
+    general-degree kernel), Q = P+1, about 2e7 cell-local dofs (at least 2048
+    cells), U, G and detJw ~ U[0, 1) from torch's seeded generator; FP64 and
+    FP32.  Reports the time per apply, cell-local GDoF/s and algorithmic GB/s
+    (U + W + geometry).  Run with --only high_degree."""
+    out = {}
+    for P in (9, 12, 16, 20, 25):
+        N = Q = P + 1
+        C = max(2048, int(2.0e7 / N ** 3))
+        gen = torch.Generator(device="cuda").manual_seed(P)
+        tab = torch.from_numpy(F.basis_tables(P, Q)).cuda()
+        U = torch.rand(C * N ** 3, device="cuda", dtype=torch.float64,
+                       generator=gen)
+        G = torch.rand(C * Q ** 3 * 6, device="cuda", dtype=torch.float64,
+                       generator=gen)
+        D = torch.rand(C * Q ** 3, device="cuda", dtype=torch.float64,
+                       generator=gen)
+        for dt, name in ((torch.float64, "f64"), (torch.float32, "f32")):
+            t, u, g, d = tab.to(dt), U.to(dt), G.to(dt), D.to(dt)
+            w = torch.empty_like(u)
+            for op in ("mass", "stiffness"):
+                def run():
+                    if op == "mass":
+                        F.mass_apply(t, u, w, d, C, P, Q)
+                    else:
+                        F.stiffness_apply(t, u, w, g, C, P, Q)
+                for _ in range(3):
+                    run()
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record()
+                for _ in range(reps):
+                    run()
+                e1.record()
+                torch.cuda.synchronize()
+                ms = e0.elapsed_time(e1) / reps
+                es = 8 if dt == torch.float64 else 4
+                geo = Q ** 3 * (1 if op == "mass" else 6)
+                out[f"P{P}_{op}_{name}"] = {
+                    "cells": C, "us": ms * 1e3,
+                    "gdofs": C * N ** 3 / (ms * 1e-3) / 1e9,
+                    "GBs": es * C * (2 * N ** 3 + geo) / (ms * 1e-3) / 1e9}
+            del t, u, g, d, w
+        del U, G, D
+        torch.cuda.empty_cache()
+    return out
+
+
 def bench_mass_c1(torch, F, sets=16, reps=8):
     """Cell-local mass apply W = M U: 32768 cells, P=3 GLL, Q=5 Gauss, U and
     W [C][64], detJw [C][125] ~ U[0.5, 1.5].  One set of buffers (66 MB)
@@ -424,8 +473,10 @@ def main():
     ap.add_argument("--apply-only", action="store_true",
                     help="skip the solve, P=4 apply, batch and serving parts")
     ap.add_argument("--only", choices=("solve", "apply_p4_64cube",
-                                       "mass_c1", "batch", "serving"),
-                    help="run the metric apply and this part only")
+                                       "mass_c1", "batch", "serving",
+                                       "high_degree"),
+                    help="run the metric apply and this part only "
+                         "(high_degree runs only when named)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
                     help="only copy MiB three times (counter calibration)")
     a = ap.parse_args()
@@ -503,8 +554,10 @@ def main():
     if not a.apply_only:
         for key, fn in (("solve", bench_solve), ("apply_p4_64cube", bench_apply_p4),
                         ("mass_c1", bench_mass_c1), ("batch", bench_batch),
-                        ("serving", bench_serving)):
-            if a.only and key != a.only:
+                        ("serving", bench_serving),
+                        ("high_degree", bench_high_degree)):
+            if (a.only and key != a.only) or (not a.only and
+                                              key == "high_degree"):
                 continue
             print(f"[bench] {key}", file=sys.stderr, flush=True)
             out[key] = fn(torch, F)
