@@ -37,7 +37,7 @@ namespace
 {
 
 constexpr int GEN_NMAX = 26; // dofs per direction
-// Workgroup shapes: BT threads, each owning one output column of up to NMAX
+// Workgroup shapes: GT threads, each owning one output column of up to NMAX
 // values in registers: (256, 16) for N <= 16, (512, 22) for N <= 22,
 // (768, 26) above.  One column per thread keeps the accumulator small, so
 // more waves stay resident.
