@@ -4,7 +4,7 @@ Host-side mirror of the reference's operator interface over the C ABI of
 libfemtc.so (include/femtc.h).  Importing the package does not load the
 library; the first call does, and fails loudly if it is not built.
 """
-from ._lib import (BF16, F32, F64, MODE_CORRECT, MODE_REFERENCE, OP_KPM,
+from ._lib import (BF16, BF16_ROUNDED, F32, F64, MODE_CORRECT, MODE_REFERENCE, OP_KPM,
                    OP_MASS, OP_STIFFNESS, SCATTER_ASSEMBLED, SCATTER_AUTO,
                    SCATTER_DIRECT, FemtcError, lib)
 from .linalg import Order, string2order
@@ -14,6 +14,7 @@ from .mesh import (basis_tables, basis_tables_weights, dofmap_structured,
 from .operators import (fill_zero, global_apply, global_apply_affine,
                         kpm_apply, mass_apply, mass_operator, stiffness_apply,
                         stiffness_operator)
+from .mfma import global_apply_mfma, mfma_tables, mfma_tables_bytes
 from .batch import Batch
 from .plan import Plan
 from .server import ResidentProblem, Server, latency_stats
@@ -22,10 +23,11 @@ from .solver import PCGSolver, global_diagonal
 __all__ = [
     "Batch", "Plan", "PCGSolver", "ResidentProblem", "Server",
     "global_diagonal", "latency_stats",
-    "BF16", "F32", "F64", "MODE_CORRECT", "MODE_REFERENCE", "OP_KPM", "OP_MASS",
+    "BF16", "BF16_ROUNDED", "F32", "F64", "MODE_CORRECT", "MODE_REFERENCE", "OP_KPM", "OP_MASS",
     "OP_STIFFNESS", "SCATTER_ASSEMBLED", "SCATTER_AUTO", "SCATTER_DIRECT",
     "FemtcError", "lib", "Order", "string2order", "fill_zero",
     "basis_tables", "dofmap_structured", "gauss_rule", "geometry_affine",
     "gll_points", "global_apply", "kpm_apply", "mass_apply", "mass_operator",
-    "stiffness_apply", "stiffness_operator",
+    "stiffness_apply", "stiffness_operator", "global_apply_mfma",
+    "mfma_tables", "mfma_tables_bytes",
 ]

@@ -22,7 +22,7 @@ ERR_NOT_CONVERGED = 8
 ERR_BREAKDOWN = 9
 ERR_NO_DEVICE = 10
 
-F64, F32, BF16 = 0, 1, 2
+F64, F32, BF16, BF16_ROUNDED = 0, 1, 2, 3
 MODE_CORRECT, MODE_REFERENCE = 0, 1
 OP_MASS, OP_STIFFNESS, OP_KPM = 0, 1, 2
 
@@ -73,6 +73,12 @@ SIGNATURES = {
                                      C.POINTER(_i64), C.POINTER(_i32)]),
     "femtc_global_apply_affine": (_i32, [_i32, _i32, _i32, _i32, _vp, _vp, _vp,
                                          _i64, _vp, _vp, _i64, _i32, _vp]),
+    "femtc_mfma_tables_bytes": (_i32, [_i32, C.POINTER(_sz)]),
+    "femtc_mfma_tables": (_i32, [_i32, _i32, _i32, _dp, _vp]),
+    "femtc_mfma_tables_host": (_i32, [_i32, _i32, _i32, _dp, _vp]),
+    "femtc_global_apply_mfma": (_i32, [_i32, _i32, _i32, _i32, _vp, _vp, _vp,
+                                       _i64, _i32, _i32, _i32, _vp, _vp, _i32,
+                                       _vp]),
     "femtc_plan_create": (_i32, [C.POINTER(_vp), _i32, _i32, _i32, _i32, _vp,
                                  _i64, _vp]),
     "femtc_plan_destroy": (_i32, [_vp]),

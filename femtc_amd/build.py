@@ -35,10 +35,17 @@ COMMON = [
 ]
 
 
+# Per-file flags.  The MFMA cell pass keeps its accumulators in VGPRs: with
+# AGPR accumulators every D tile costs four v_accvgpr_read before its values
+# can be converted and stored.
+EXTRA = {"mfma_cell.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"]}
+
+
 def _compile(src: str) -> tuple[str, float]:
     obj = os.path.join(BUILD, os.path.basename(src) + ".o")
     t0 = time.time()
-    cmd = [HIPCC, *COMMON, "-c", src, "-o", obj]
+    cmd = [HIPCC, *COMMON, *EXTRA.get(os.path.basename(src), []), "-c", src,
+           "-o", obj]
     if src.endswith(".cpp"):
         cmd = [HIPCC, *COMMON, "-x", "hip", "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -159,6 +159,33 @@ int launch_apply_n9(const ApplyReq&);
 // N = 10..26 (apply_general.hip); also accepts smaller N
 int launch_apply_general(const ApplyReq&);
 
+// The K+M cell pass with bfloat16 operands on the matrix cores and FP32
+// accumulation (mfma_cell.hip): affine cells of a structured ncx x ncy x ncz
+// grid, Q == N <= 8, FP32 x in, cell values stored into the assembled
+// scatter's lattice (assemble.hpp), one x_e^T A_e x_e partial per cell.
+// ks and km scale the stiffness and the mass term (1 or 0: K+M, K or M).
+struct MfmaReq
+{
+  int N = 0, Q = 0;
+  int split = 1;               // 1: values split hi + lo as well as tables
+  const void* amat = nullptr;  // operand set of mfma_build_tables (device)
+  const float* w = nullptr;    // the Q Gauss weights (device)
+  const float* x = nullptr;
+  float* lattice = nullptr;
+  const float* cg = nullptr;   // [C][7]: G's packing of detJ J^-1 J^-T, detJ
+  float ks = 1.f, km = 1.f;
+  int ncx = 0, ncy = 0, ncz = 0;
+  const int* skip = nullptr;
+  double* energy = nullptr;
+  hipStream_t stream = nullptr;
+};
+int launch_mfma(const MfmaReq& r);
+// Matrices of the operand set, and the set itself: nmat * 64 lanes * 8
+// bfloat16 values from the FP64 host tables [phi | phi'] ([2][N][Q]).
+int mfma_table_matrices(int split);
+int mfma_build_tables(const double* tab, int N, int Q, int split,
+                      uint16_t* out);
+
 inline int hip_status(hipError_t e)
 {
   return e == hipSuccess ? FEMTC_OK : FEMTC_ERR_HIP;

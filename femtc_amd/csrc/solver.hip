@@ -650,6 +650,11 @@ struct femtc_solver
   bool even_odd = false;
   void* eo64 = nullptr;
   void* eo_in = nullptr;
+  // inner operator on the matrix cores (inner_dtype FEMTC_BF16 or
+  // FEMTC_BF16_ROUNDED): -1 when not, else the MfmaReq split flag; its
+  // operand set (tables as A operands, then the weights) on the device
+  int mfma_split = -1;
+  void* mfma_ops = nullptr;
   // the batch of check_every iterations as a hipGraph (captured on
   // cap_stream at the first solve, replayed on `stream`)
   bool use_graph = true;
@@ -721,6 +726,7 @@ void destroy(femtc_solver* S)
   dev_free(S->cg64);
   dev_free(S->epart);
   dev_free(S->lattice);
+  dev_free(S->mfma_ops);
   if (S->graph)
     (void)hipGraphExecDestroy(S->graph);
   if (S->cap_stream)
@@ -786,6 +792,33 @@ ApplyReq outer_req(const femtc_solver* S, const void* x, void* y)
   }
   return operator_req(S, FEMTC_F64, S->prob.tables, S->prob.G, S->prob.detJw,
                       x, y);
+}
+
+// The inner operator of request a: the fused kernels, or the cell pass on
+// the matrix cores into the lattice (the CG update assembles it).
+int run_inner(const femtc_solver* S, const ApplyReq& a)
+{
+  if (S->mfma_split < 0)
+    return run(a);
+  MfmaReq m;
+  m.N = m.Q = S->N;
+  m.split = S->mfma_split;
+  m.amat = S->mfma_ops;
+  m.w = reinterpret_cast<const float*>(
+      static_cast<const unsigned char*>(S->mfma_ops) +
+      (size_t)mfma_table_matrices(S->mfma_split) * 64 * 16);
+  m.x = static_cast<const float*>(a.x);
+  m.lattice = static_cast<float*>(S->lattice);
+  m.cg = static_cast<const float*>(S->G);
+  m.ks = S->prob.op == FEMTC_OP_MASS ? 0.f : 1.f;
+  m.km = S->prob.op == FEMTC_OP_STIFFNESS ? 0.f : 1.f;
+  m.ncx = S->prob.grid[0];
+  m.ncy = S->prob.grid[1];
+  m.ncz = S->prob.grid[2];
+  m.skip = a.skip;
+  m.energy = a.energy;
+  m.stream = a.stream;
+  return launch_mfma(m);
 }
 
 LatticeGrid lattice_grid(const femtc_solver* S)
@@ -854,7 +887,7 @@ int time_step(femtc_solver* S, int reps, float* ms)
     if (i == 1 && hipEventRecord(e0, S->stream) != hipSuccess)
       st = FEMTC_ERR_HIP;
     if (st == FEMTC_OK)
-      st = run(a);
+      st = run_inner(S, a);
     if (st == FEMTC_OK)
       st = launch_update<V>(S, S->stream);
   }
@@ -891,6 +924,20 @@ int64_t energy_partials(int64_t C, int Q, bool assembled, int dtype)
 // alpha = 0 for the timing; the solve re-initialises everything).
 int choose_variant(femtc_solver* S, int structured)
 {
+  if (S->mfma_split >= 0)
+  {
+    // the matrix-core cell pass writes the lattice: assembled scatter only
+    if (!structured)
+      return fail(FEMTC_ERR_INVALID_ARGUMENT,
+                  "BF16 inner operator: the dofmap is not the structured "
+                  "numbering of the grid");
+    S->inner_assembled = true;
+    S->even_odd = false;
+    S->nepart = S->prob.num_cells; // one energy partial per cell (wave)
+    return dev_alloc(&S->lattice,
+                     4 * (size_t)lattice_size(S->prob.grid[0], S->prob.grid[1],
+                                              S->prob.grid[2], S->N - 1));
+  }
   const int mode = S->prob.scatter;
   const bool have_eo = S->eo64 && S->eo_in;
   bool try_asm = structured && mode != FEMTC_SCATTER_DIRECT;
@@ -990,7 +1037,7 @@ int enqueue_batch(femtc_solver* S, hipStream_t s)
   a.stream = s;
   for (int k = 0; k < S->check_every; ++k)
   {
-    int st = run(a);
+    int st = run_inner(S, a);
     if (st != FEMTC_OK)
       return st;
     hipLaunchKernelGGL(k_energy_partials, dim3(NBLK), dim3(RB), 0, s,
@@ -1199,13 +1246,21 @@ int femtc_solver_create(femtc_solver** out, const femtc_problem* prob,
   int st = check_problem(prob);
   if (st != FEMTC_OK)
     return st;
-  if (inner_dtype != FEMTC_F64 && inner_dtype != FEMTC_F32)
+  const bool bf = inner_dtype == FEMTC_BF16 || inner_dtype == FEMTC_BF16_ROUNDED;
+  if (inner_dtype != FEMTC_F64 && inner_dtype != FEMTC_F32 && !bf)
     return fail(FEMTC_ERR_INVALID_ARGUMENT,
-                "inner_dtype must be FEMTC_F64 (plain PCG) or FEMTC_F32 "
-                "(FP32 inner PCG with FP64 defect correction)");
+                "inner_dtype must be FEMTC_F64 (plain PCG), FEMTC_F32, "
+                "FEMTC_BF16 or FEMTC_BF16_ROUNDED (inner PCG with FP64 defect "
+                "correction)");
+  if (bf && (!prob->J || prob->grid[0] < 1 || prob->nq != prob->degree + 1 ||
+             prob->degree > 7))
+    return fail(FEMTC_ERR_INVALID_ARGUMENT,
+                "BF16 inner operator: needs the cell Jacobians J, the "
+                "structured grid, nq = degree + 1 and degree <= 7");
   femtc_solver* S = new femtc_solver;
   S->prob = *prob;
   S->inner_dtype = inner_dtype;
+  S->mfma_split = bf ? (inner_dtype == FEMTC_BF16 ? 1 : 0) : -1;
   S->N = prob->degree + 1;
   S->Q = prob->nq;
   S->n = prob->ndofs;
@@ -1230,8 +1285,9 @@ int femtc_solver_create(femtc_solver** out, const femtc_problem* prob,
       break;
     // one energy partial per wave of the operator launch, whose cell
     // groups depend on the scatter (set again once it is chosen)
-    S->nepart = std::max(energy_partials(C, S->Q, false, inner_dtype),
-                         energy_partials(C, S->Q, true, inner_dtype));
+    S->nepart = bf ? C
+                   : std::max(energy_partials(C, S->Q, false, inner_dtype),
+                              energy_partials(C, S->Q, true, inner_dtype));
     if ((st = dev_alloc((void**)&S->epart, 8 * S->nepart)))
       break;
     if (hipHostMalloc((void**)&S->hst, sizeof(PcgState)) != hipSuccess ||
@@ -1332,6 +1388,18 @@ int femtc_solver_create(femtc_solver** out, const femtc_problem* prob,
                            0, s, prob->detJw, (float*)S->detJw, Q3 * C);
       }
     }
+    if (bf)
+    {
+      // the matrix-core operand set, from the FP64 tables read back once
+      std::vector<double> th(ntab);
+      size_t ob = 0;
+      if ((st = femtc_mfma_tables_bytes(S->mfma_split, &ob)) ||
+          (st = sync_read(S, th.data(), prob->tables, 8 * ntab)) ||
+          (st = dev_alloc(&S->mfma_ops, ob)) ||
+          (st = femtc_mfma_tables(prob->degree, prob->nq, S->mfma_split,
+                                  th.data(), S->mfma_ops)))
+        break;
+    }
     hipError_t he = hipGetLastError();
     if (he != hipSuccess)
     {
@@ -1352,7 +1420,7 @@ int femtc_solver_create(femtc_solver** out, const femtc_problem* prob,
     // even-odd table sets when the tables are symmetric (the weights of the
     // Gauss rule ride along for the per-cell geometry operators); whether
     // they are used is timed below with the scatter
-    if (prob->even_odd != FEMTC_EVEN_ODD_OFF && S->Q == S->N)
+    if (prob->even_odd != FEMTC_EVEN_ODD_OFF && S->Q == S->N && !bf)
     {
       std::vector<double> xq(S->Q), wq(S->Q);
       femtc_gauss_rule(S->Q, xq.data(), wq.data());

@@ -2,8 +2,9 @@
 
 The reference has no solver; this is the solve path built on the operator:
 A = M, K or K+M applied through the dofmap, a Jacobi preconditioner from the
-sum-factorized diagonal, CG scalars kept on the device, and (inner=F32) an
-FP32 inner PCG wrapped in FP64 iterative refinement.
+sum-factorized diagonal, CG scalars kept on the device, and (inner=F32,
+BF16 or BF16_ROUNDED) an inner PCG in FP32 vectors wrapped in FP64 iterative
+refinement, its operator in FP32 or on the matrix cores in bfloat16.
 """
 from __future__ import annotations
 
@@ -27,8 +28,10 @@ class PCGSolver:
     """Solver for A u = f on one problem (device tensors, FP64).
 
     inner=F64: plain PCG.  inner=F32: FP32 inner PCG (tolerance inner_rtol)
-    inside FP64 defect correction.  The tensors passed here must stay alive
-    and unchanged while the solver is used."""
+    inside FP64 defect correction.  inner=BF16 / BF16_ROUNDED: the same with
+    the inner operator on the matrix cores (bfloat16 operands, FP32
+    accumulation; needs J and grid, nq = degree + 1 <= 8).  The tensors
+    passed here must stay alive and unchanged while the solver is used."""
 
     def __init__(self, op: int, tables, dofmap, num_cells: int, degree: int,
                  nq: int, ndofs: int, G=None, detJw=None, inner: int = _lib.F64,

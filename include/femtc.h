@@ -56,7 +56,15 @@ typedef enum femtc_dtype
 {
   FEMTC_F64 = 0,
   FEMTC_F32 = 1,
-  FEMTC_BF16 = 2
+  /* Inner precisions of a refined solve only (femtc_solver_create): the
+   * operator with bfloat16 operands on the matrix cores and FP32
+   * accumulation (femtc_global_apply_mfma), FP32 Krylov vectors.
+   * FEMTC_BF16 splits the tables and the values (split = 1, about 1e-5);
+   * FEMTC_BF16_ROUNDED rounds the values (split = 0, about 3e-3: pair it
+   * with an inner tolerance near 3e-3, tighter solves of the perturbed
+   * operator buy nothing). */
+  FEMTC_BF16 = 2,
+  FEMTC_BF16_ROUNDED = 3
 } femtc_dtype;
 
 /* linalg::Order, same enumerators in the same order (src/linalg.hpp:11-19).
@@ -202,6 +210,42 @@ int femtc_global_apply_affine(int op, int dtype, int degree, int nq,
                               int64_t ndofs, const int32_t* dofmap,
                               const void* cellgeo, int64_t num_cells,
                               int accumulate, void* stream);
+
+/* ---------------------------------------------------------------------- */
+/* The same operators with bfloat16 operands on the matrix cores (MFMA)    */
+/* and FP32 accumulation: the operator of the BF16 inner solve.  No        */
+/* counterpart in the reference, whose contraction chain                   */
+/* (src/stiffness.hpp:68-97, src/mass.hpp:36-47) runs in T = float or      */
+/* double.  Affine cells of a structured nx x ny x nz grid (cell (i,j,k)   */
+/* is (i*ny + j)*nz + k, dofs numbered as femtc_dofmap_structured numbers  */
+/* them, so no dofmap is read), nq = degree + 1, degree 1..7, FP32         */
+/* vectors.  Every 1-D contraction is a bfloat16 MFMA product with FP32    */
+/* accumulation.  The tables are split into two bfloat16 parts (hi + lo);  */
+/* split = 1 splits the cell values too (products hi.hi + hi.lo + lo.hi:   */
+/* about 1e-5 relative to the FP64 operator), split = 0 rounds them to     */
+/* bfloat16 once per contraction (about 3e-3).                             */
+/* ---------------------------------------------------------------------- */
+/* Bytes of the operand set (the tables as MFMA operands, and the weights). */
+int femtc_mfma_tables_bytes(int split, size_t* bytes);
+/* The operand set built on the host into host memory of
+ * femtc_mfma_tables_bytes bytes: for each matrix m, lane l, slot s the
+ * bfloat16 A[l % 16][8 (l / 16) + s] at ((m * 64 + l) * 8 + s), then the nq
+ * Gauss weights as 8 floats.  No device work. */
+int femtc_mfma_tables_host(int degree, int nq, int split,
+                           const double* tables, void* operands);
+/* The same, from the host tables [phi | dphi] (FP64, 2*N*nq values) into
+ * device memory of femtc_mfma_tables_bytes bytes.  Synchronous. */
+int femtc_mfma_tables(int degree, int nq, int split, const double* tables,
+                      void* operands);
+/* y = A x (accumulate = 0) or y += A x, op a femtc_op; cellgeo [C][7] in
+ * FP32 as femtc_geometry_cell writes it; lattice: a device workspace of
+ * C*(degree+1)^3 floats that receives the cell values, which a second
+ * kernel sums per dof (no atomics: the result is deterministic). */
+int femtc_global_apply_mfma(int op, int degree, int nq, int split,
+                            const void* operands, const float* x, float* y,
+                            int64_t ndofs, int nx, int ny, int nz,
+                            const float* cellgeo, float* lattice,
+                            int accumulate, void* stream);
 
 /* ---------------------------------------------------------------------- */
 /* Reference-shaped synchronous entry points on HOST buffers: the C side   */
@@ -436,9 +480,13 @@ typedef struct femtc_solver femtc_solver; /* opaque */
 /* Allocate the workspace on the current device and build the Jacobi
  * preconditioner from the FP64 operator.  inner_dtype FEMTC_F64: plain PCG
  * in FP64.  FEMTC_F32: FP32 inner PCG (FP32 vectors, tables and geometry
- * copy; FP64 dot products) inside FP64 defect correction.  The solver and
- * every later solve run on `stream`.  Fails with FEMTC_ERR_INVALID_ARGUMENT
- * when a diagonal entry is not positive (a dof no cell touches). */
+ * copy; FP64 dot products) inside FP64 defect correction.  FEMTC_BF16 and
+ * FEMTC_BF16_ROUNDED: the same with the inner operator on the matrix cores
+ * (bfloat16 operands, FP32 accumulation); they need the cell Jacobians J,
+ * the structured grid (and a dofmap that is its structured numbering),
+ * nq = degree + 1 and degree <= 7.  The solver and every later solve run on
+ * `stream`.  Fails with FEMTC_ERR_INVALID_ARGUMENT when a diagonal entry is
+ * not positive (a dof no cell touches). */
 int femtc_solver_create(femtc_solver** solver, const femtc_problem* problem,
                         int inner_dtype, void* stream);
 /* Solve A u = f (f, u FP64 device vectors of ndofs; u overwritten) to
@@ -489,9 +537,9 @@ typedef struct femtc_request
   const void* input;            /* FP64 device vector of ndofs */
   void* output;                 /* FP64 device vector of ndofs */
   double rtol;                  /* solve */
-  double inner_rtol;            /* solve with inner_dtype FEMTC_F32 */
+  double inner_rtol;            /* refined solves (inner_dtype not F64) */
   int maxit;                    /* solve */
-  int inner_dtype;              /* solve: FEMTC_F64 or FEMTC_F32 */
+  int inner_dtype;              /* solve: as femtc_solver_create's */
   int64_t tag;                  /* returned with the result */
 } femtc_request;
 
