@@ -79,6 +79,9 @@ SIGNATURES = {
     "femtc_global_apply_mfma": (_i32, [_i32, _i32, _i32, _i32, _vp, _vp, _vp,
                                        _i64, _i32, _i32, _i32, _vp, _vp, _i32,
                                        _vp]),
+    "femtc_global_apply_mfma_noalloc": (_i32, [_i32, _i32, _i32, _i32, _vp,
+                                               _vp, _vp, _i64, _i32, _i32,
+                                               _i32, _vp, _i32, _vp]),
     "femtc_plan_create": (_i32, [C.POINTER(_vp), _i32, _i32, _i32, _i32, _vp,
                                  _i64, _vp]),
     "femtc_plan_destroy": (_i32, [_vp]),

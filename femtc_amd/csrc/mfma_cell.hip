@@ -46,6 +46,8 @@
 
 #include <cmath>
 #include <cstring>
+#include <mutex>
+#include <string>
 #include <vector>
 
 namespace femtc
@@ -784,11 +786,15 @@ int femtc_mfma_tables(int degree, int nq, int split, const double* tables,
   return FEMTC_OK;
 }
 
-int femtc_global_apply_mfma(int op, int degree, int nq, int split,
-                            const void* operands, const float* x, float* y,
-                            int64_t ndofs, int nx, int ny, int nz,
-                            const float* cellgeo, float* lattice,
-                            int accumulate, void* stream)
+} // extern "C"
+
+namespace
+{
+// Arguments of femtc_global_apply_mfma[_noalloc] other than the lattice.
+int check_mfma_args(int op, int degree, int nq, int split,
+                    const void* operands, const float* x, const float* y,
+                    int64_t ndofs, int nx, int ny, int nz,
+                    const float* cellgeo)
 {
   if (op != FEMTC_OP_MASS && op != FEMTC_OP_STIFFNESS && op != FEMTC_OP_KPM)
     return fail(FEMTC_ERR_INVALID_ARGUMENT, "op must be a femtc_op");
@@ -808,8 +814,26 @@ int femtc_global_apply_mfma(int op, int degree, int nq, int split,
   if (ndofs != nd)
     return fail(FEMTC_ERR_INVALID_ARGUMENT,
                 "ndofs is not the dof count of the structured grid");
-  if (!operands || !x || !y || !cellgeo || !lattice)
+  if (!operands || !x || !y || !cellgeo)
     return fail(FEMTC_ERR_INVALID_ARGUMENT, "null argument");
+  return FEMTC_OK;
+}
+} // namespace
+
+extern "C" {
+
+int femtc_global_apply_mfma(int op, int degree, int nq, int split,
+                            const void* operands, const float* x, float* y,
+                            int64_t ndofs, int nx, int ny, int nz,
+                            const float* cellgeo, float* lattice,
+                            int accumulate, void* stream)
+{
+  int st = check_mfma_args(op, degree, nq, split, operands, x, y, ndofs, nx,
+                           ny, nz, cellgeo);
+  if (st != FEMTC_OK)
+    return st;
+  if (!lattice)
+    return fail(FEMTC_ERR_INVALID_ARGUMENT, "null lattice workspace");
   hipStream_t s = (hipStream_t)stream;
   MfmaReq r;
   r.N = r.Q = degree + 1;
@@ -827,7 +851,7 @@ int femtc_global_apply_mfma(int op, int degree, int nq, int split,
   r.ncy = ny;
   r.ncz = nz;
   r.stream = s;
-  int st = launch_mfma(r);
+  st = launch_mfma(r);
   if (st != FEMTC_OK)
     return fail(st, "MFMA cell pass launch failed");
   st = assemble_f32(lattice, y, LatticeGrid{nx, ny, nz, degree}, accumulate, s);
@@ -835,3 +859,65 @@ int femtc_global_apply_mfma(int op, int degree, int nq, int split,
 }
 
 } // extern "C"
+
+// ---- the same apply without allocation: a static scratch lattice --------
+// The lattice of femtc_global_apply_mfma_noalloc lives in this file-scope
+// device array, one copy per GPU the library runs on, held until the
+// process ends.  Host code never uses its name as an address: the device
+// address comes from hipGetSymbolAddress, once per device.
+namespace
+{
+constexpr long long SCRATCH_FLOATS = 16LL << 20; // 64 MiB
+static __device__ __align__(256) float scratch[SCRATCH_FLOATS];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex scratch_mutex;
+float* scratch_ptr[MAX_DEVICES] = {};
+
+// The device address of `scratch` on the current device (resolved at the
+// first call on that device).
+int scratch_address(float** out)
+{
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAX_DEVICES)
+    return fail(FEMTC_ERR_NO_DEVICE, "no current HIP device");
+  std::lock_guard<std::mutex> lock(scratch_mutex);
+  if (!scratch_ptr[dev])
+  {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(FEMTC_ERR_HIP, std::string("scratch symbol address: ") +
+                                     hipGetErrorString(e));
+    scratch_ptr[dev] = static_cast<float*>(p);
+  }
+  *out = scratch_ptr[dev];
+  return FEMTC_OK;
+}
+} // namespace
+
+extern "C" int femtc_global_apply_mfma_noalloc(
+    int op, int degree, int nq, int split, const void* operands,
+    const float* x, float* y, int64_t ndofs, int nx, int ny, int nz,
+    const float* cellgeo, int accumulate, void* stream)
+{
+  int st = check_mfma_args(op, degree, nq, split, operands, x, y, ndofs, nx,
+                           ny, nz, cellgeo);
+  if (st != FEMTC_OK)
+    return st;
+  const long long need = (long long)nx * ny * nz * (degree + 1) *
+                         (degree + 1) * (degree + 1);
+  if (need > SCRATCH_FLOATS)
+    return fail(FEMTC_ERR_INVALID_ARGUMENT,
+                "femtc_global_apply_mfma_noalloc: the grid needs " +
+                    std::to_string(need) +
+                    " lattice values; the static scratch holds " +
+                    std::to_string(SCRATCH_FLOATS));
+  float* lat = nullptr;
+  if ((st = scratch_address(&lat)) != FEMTC_OK)
+    return st;
+  // every lattice value is written by the cell pass before the assembly
+  // reads it, so the scratch needs no clearing between calls
+  return femtc_global_apply_mfma(op, degree, nq, split, operands, x, y, ndofs,
+                                 nx, ny, nz, cellgeo, lat, accumulate, stream);
+}

@@ -42,3 +42,19 @@ def global_apply_mfma(op: int, degree: int, nq: int, operands, x, y, grid,
         op, degree, nq, int(split), _ptr(operands), _ptr(x), _ptr(y),
         y.numel(), nx, ny, nz, _ptr(cellgeo), _ptr(lattice), int(accumulate),
         _stream(stream)), "global_apply_mfma")
+
+
+NOALLOC_LATTICE_VALUES = 16 << 20
+
+
+def global_apply_mfma_noalloc(op: int, degree: int, nq: int, operands, x, y,
+                              grid, cellgeo, split: int = 1,
+                              accumulate: bool = False, stream=None) -> None:
+    """global_apply_mfma without a workspace argument and without allocating:
+    the lattice is the library's static 64 MiB device array (grids of at most
+    NOALLOC_LATTICE_VALUES cell values); overlapping calls on one GPU race."""
+    nx, ny, nz = (int(v) for v in grid)
+    _lib.check(_lib.lib().femtc_global_apply_mfma_noalloc(
+        op, degree, nq, int(split), _ptr(operands), _ptr(x), _ptr(y),
+        y.numel(), nx, ny, nz, _ptr(cellgeo), int(accumulate),
+        _stream(stream)), "global_apply_mfma_noalloc")

@@ -246,6 +246,19 @@ int femtc_global_apply_mfma(int op, int degree, int nq, int split,
                             int64_t ndofs, int nx, int ny, int nz,
                             const float* cellgeo, float* lattice,
                             int accumulate, void* stream);
+/* The same apply for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid it); no
+ * counterpart in the reference.  It calls no allocating routine: the
+ * lattice is a static device array of the library, 16 Mi floats (64 MiB)
+ * per GPU it runs on, held until the process ends, so nx*ny*nz*(degree+1)^3
+ * must not exceed 16777216 (FEMTC_ERR_INVALID_ARGUMENT otherwise).  Every
+ * call on a GPU shares that GPU's array: calls that overlap on one GPU (other
+ * streams, other threads) race.  Results equal femtc_global_apply_mfma's. */
+int femtc_global_apply_mfma_noalloc(int op, int degree, int nq, int split,
+                                    const void* operands, const float* x,
+                                    float* y, int64_t ndofs, int nx, int ny,
+                                    int nz, const float* cellgeo,
+                                    int accumulate, void* stream);
 
 /* ---------------------------------------------------------------------- */
 /* Reference-shaped synchronous entry points on HOST buffers: the C side   */
