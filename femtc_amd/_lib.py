@@ -104,6 +104,8 @@ SIGNATURES = {
     "femtc_batch_set_group": (_i32, [_vp, _i32, _i32]),
     "femtc_batch_destroy": (_i32, [_vp]),
     "femtc_server_create": (_i32, [C.POINTER(_vp), _i32, _sz]),
+    "femtc_server_register": (_i32, [_vp, _vp]),
+    "femtc_server_unregister": (_i32, [_vp, _vp]),
     "femtc_server_submit": (_i32, [_vp, _vp]),
     "femtc_server_drain": (_i32, [_vp]),
     "femtc_server_results": (_i32, [_vp, _vp, _i32, C.POINTER(_i32)]),
@@ -119,6 +121,10 @@ SIGNATURES = {
     "femtc_solver_even_odd": (_i32, [_vp, C.POINTER(_i32)]),
     "femtc_solver_set_batch": (_i32, [_vp, _i32, _i32]),
     "femtc_solver_destroy": (_i32, [_vp]),
+    "femtc_solver_bytes": (_i32, [_vp, C.POINTER(_sz)]),
+    "femtc_solver_workspace_bytes": (_i32, [_vp, _i32, C.POINTER(_sz)]),
+    "femtc_admission_simulate": (_i32, [_vp, _i32, _i32, _vp, _i32, _sz, _vp,
+                                        _vp]),
 }
 
 
@@ -159,6 +165,26 @@ class RequestResult(C.Structure):
                 ("outer_iterations", _i32), ("relres", C.c_double),
                 ("t_arrival", C.c_double), ("t_start", C.c_double),
                 ("t_end", C.c_double)]
+
+
+class SimRequest(C.Structure):
+    """femtc_sim_request (include/femtc.h)."""
+    _fields_ = [("t_arrival", C.c_double), ("kind", _i32), ("problem", _i32),
+                ("inner", _i32), ("bytes", _sz), ("actual", _sz),
+                ("service", C.c_double), ("create", C.c_double)]
+
+
+class SimResult(C.Structure):
+    """femtc_sim_result (include/femtc.h)."""
+    _fields_ = [("status", _i32), ("slot", _i32), ("cached", _i32),
+                ("t_start", C.c_double), ("t_end", C.c_double)]
+
+
+class SimStats(C.Structure):
+    """femtc_sim_stats (include/femtc.h)."""
+    _fields_ = [("high_water", _sz), ("evictions", _i64), ("created", _i64),
+                ("destroyed", _i64), ("makespan", C.c_double),
+                ("left_in_queue", _i64)]
 
 
 class SolveInfo(C.Structure):

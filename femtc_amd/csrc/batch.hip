@@ -295,6 +295,24 @@ int femtc_batch_create(femtc_batch** out, const femtc_batch_item* items,
     if (st != FEMTC_OK)
       return st;
   }
+  // every dofmap entry in range before a kernel scatters through it
+  for (int i = 0; i < nitems; ++i)
+  {
+    const femtc_batch_item& it = items[i];
+    if (!it.dofmap || it.num_cells == 0)
+      continue;
+    const int64_t n3 = (int64_t)(it.degree + 1) * (it.degree + 1) *
+                       (it.degree + 1);
+    int ok = 0;
+    const int st = check_dofmap_range(it.dofmap, it.num_cells * n3, it.ndofs,
+                                      (hipStream_t)stream, &ok);
+    if (st != FEMTC_OK)
+      return st;
+    if (!ok)
+      return fail(FEMTC_ERR_INVALID_ARGUMENT,
+                  "batch item " + std::to_string(i) +
+                      ": a dofmap entry is outside [0, ndofs)");
+  }
   femtc_batch* b = new femtc_batch;
   b->stream = (hipStream_t)stream;
   // group by (op, dtype, N, Q, gather)

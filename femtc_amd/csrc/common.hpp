@@ -206,6 +206,11 @@ int fill_zero(void* y, int64_t n, int dtype, hipStream_t s);
 int verify_structured(const int32_t* dofmap, int nx, int ny, int nz,
                       int degree, int64_t ndofs, hipStream_t s, int* ok);
 
+// *ok = 1 when every one of the n dofmap entries is in [0, ndofs), before
+// any kernel indexes through them.  Synchronous on `s` (plan.hip).
+int check_dofmap_range(const int32_t* dofmap, int64_t n, int64_t ndofs,
+                       hipStream_t s, int* ok);
+
 // Reads the device tables [phi | phi'] ([2][N][Q] in `dtype`) on `s`; when
 // they pass eo_symmetric, allocates and fills *eo with their even-odd table
 // set (even_odd.hpp) in `out_dtype`, with the weights w (may be null),

@@ -182,6 +182,57 @@ int verify_structured(const int32_t* dofmap, int nx, int ny, int nz,
   return st;
 }
 
+namespace
+{
+__global__ void __launch_bounds__(RB)
+    k_dofmap_range(const int32_t* __restrict__ dofmap, int64_t n,
+                   int64_t ndofs, int* __restrict__ bad)
+{
+  for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * RB)
+  {
+    const int32_t g = dofmap[i];
+    if (g < 0 || g >= ndofs)
+      bad[0] = 1;
+  }
+}
+} // namespace
+
+int check_dofmap_range(const int32_t* dofmap, int64_t n, int64_t ndofs,
+                       hipStream_t s, int* ok)
+{
+  *ok = 0;
+  if (n <= 0)
+  {
+    *ok = 1;
+    return FEMTC_OK;
+  }
+  if (!dofmap)
+    return fail(FEMTC_ERR_INVALID_ARGUMENT, "dofmap is NULL");
+  int* bad = nullptr;
+  if (hipMalloc((void**)&bad, sizeof(int)) != hipSuccess)
+  {
+    (void)hipGetLastError();
+    return fail(FEMTC_ERR_OUT_OF_MEMORY, "dofmap check allocation failed");
+  }
+  int hbad = 1;
+  int st = FEMTC_OK;
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(RB), 0, s, (uint32_t*)bad,
+                     (int64_t)1);
+  hipLaunchKernelGGL(k_dofmap_range, dim3(grid_for(n)), dim3(RB), 0, s,
+                     dofmap, n, ndofs, bad);
+  if (hipGetLastError() != hipSuccess ||
+      hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, s) !=
+          hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    st = fail(FEMTC_ERR_HIP, std::string("dofmap range check: ") +
+                                 hipGetErrorString(hipGetLastError()));
+  (void)hipFree(bad);
+  if (st == FEMTC_OK)
+    *ok = hbad ? 0 : 1;
+  return st;
+}
+
 int make_even_odd(const void* tables, int dtype, int N, int Q, const double* w,
                   int out_dtype, hipStream_t s, void** eo)
 {

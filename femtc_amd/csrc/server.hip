@@ -1,19 +1,23 @@
-// Request server: apply and solve requests on resident problems, served by
+// Request server: apply and solve requests on registered problems, served by
 // a fixed number of slots (host threads, one HIP stream each) so that work
 // admitted while other requests are running overlaps on the device
 // (continuous batching).  Solve requests keep their solver (Jacobi diagonal,
 // Krylov and refinement vectors, lower-precision geometry copy) resident per
-// (problem, inner precision, slot); admission counts those workspaces
-// against an HBM budget and evicts idle ones least-recently-used first.  A
-// request that does not fit waits while later ones that do fit go ahead.
+// (problem, inner precision, slot).  Admission, queueing and eviction are the
+// host-only Admission class (admission.hpp), run here under the server's
+// lock with real solvers and the real clock: a request that does not fit
+// waits while later ones that do fit go ahead, idle solvers are evicted
+// least-recently-used first, and they are destroyed after the lock is
+// released, so a slot's bookkeeping never waits on another's hipFree.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
-#include <deque>
 #include <mutex>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
+#include "admission.hpp"
 #include "assemble.hpp"
 #include "common.hpp"
 
@@ -24,24 +28,28 @@ namespace
 
 using Clock = std::chrono::steady_clock;
 
+// Estimated device bytes of a solver of problem p (femtc_solver_create's
+// allocations); the admission reserves this much until the solver exists
+// and then charges what femtc_solver_bytes reports.
 size_t solver_bytes(const femtc_problem& p, int inner)
 {
   const size_t n = (size_t)p.ndofs;
   const size_t q3 = (size_t)p.nq * p.nq * p.nq;
   const size_t C = (size_t)p.num_cells;
-  size_t b = 4 * 8 * n;                           // diag, dinv, r, A u (FP64)
+  const size_t N3 = (size_t)(p.degree + 1) * (p.degree + 1) * (p.degree + 1);
+  size_t b = 4 * 8 * n;                    // diag, D^-1, r, A u (FP64)
   const size_t vs = inner == FEMTC_F64 ? 8 : 4;
-  b += 4 * vs * n;                                // rhs, e, p, A p
+  b += 4 * vs * n;                         // rhs, e, p, A p
+  b += 8 * C * 4;                          // energy partials (<= 4 per cell)
   if (p.J)
-    b += 8 * CELLGEO * C;                         // per-cell geometry, FP64
-  if (p.grid[0] > 0)                              // assembled-scatter lattice
-    b += vs * (size_t)lattice_size(p.grid[0], p.grid[1], p.grid[2],
-                                   p.degree);
+    b += 8 * CELLGEO * C;                  // per-cell geometry, FP64
+  if (p.grid[0] > 0)                       // assembled-scatter lattice
+    b += vs * C * N3;
   if (inner != FEMTC_F64)
   {
-    b += 2 * 4 * n;                               // r, D^-1 (FP32)
+    b += (4 + 2 + 4) * n;                  // r, D^-1 (bfloat16 and FP32)
     if (p.J)
-      b += 4 * CELLGEO * C;                       // per-cell geometry, FP32
+      b += 4 * CELLGEO * C;                // per-cell geometry, FP32
     else
     {
       if (p.G)
@@ -50,7 +58,7 @@ size_t solver_bytes(const femtc_problem& p, int inner)
         b += 4 * q3 * C;
     }
   }
-  return b + 64 * 1024; // partial sums, scalars, tables
+  return b + (1 << 20); // partial sums, scalars, tables, operand sets
 }
 
 } // namespace
@@ -60,31 +68,28 @@ using namespace femtc;
 
 struct femtc_server
 {
-  struct Pending
+  struct Job
   {
     femtc_request r;
+    femtc_problem p; // the registered copy
+    uint64_t key;
     double t_arrival;
   };
-  struct Cached
+  struct Registered
   {
-    const femtc_problem* problem;
-    int inner;
-    int slot;
-    femtc_solver* solver;
-    size_t bytes;
-    uint64_t last_use;
-    bool busy;
+    uint64_t key;
+    femtc_problem copy;
   };
   int slots = 1;
   int device = 0;
-  size_t budget = 0;
-  size_t used = 0;
-  uint64_t tick = 0;
+  Admission adm{0};
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
-  std::deque<Pending> queue;
+  std::unordered_map<int64_t, Job> jobs; // queued and running requests
+  int64_t next_id = 0;
+  std::unordered_map<const femtc_problem*, Registered> reg;
+  std::unordered_map<uint64_t, int> running; // requests running per problem
   std::vector<femtc_request_result> done;
-  std::vector<Cached> cache;
   int inflight = 0;
   bool stop = false;
   std::vector<std::thread> workers;
@@ -95,34 +100,31 @@ struct femtc_server
     return std::chrono::duration<double>(Clock::now() - t0).count();
   }
 
-  // Index of a cached solver for (p, inner, slot), or -1.
-  int find(const femtc_problem* p, int inner, int slot) const
+  static void destroy_all(std::vector<void*>& v)
   {
-    for (size_t i = 0; i < cache.size(); ++i)
-      if (cache[i].problem == p && cache[i].inner == inner &&
-          cache[i].slot == slot)
-        return (int)i;
-    return -1;
+    for (void* h : v)
+      femtc_solver_destroy(static_cast<femtc_solver*>(h));
+    v.clear();
   }
 
-  // Make room for `need` bytes by evicting idle solvers (LRU).  Called with
-  // the lock held; returns false if the budget cannot be met now.
-  bool make_room(size_t need)
+  // Results of requests dropped with their problem (lock held).
+  void drop(const std::vector<int64_t>& ids)
   {
-    while (used + need > budget)
+    for (int64_t id : ids)
     {
-      int victim = -1;
-      for (size_t i = 0; i < cache.size(); ++i)
-        if (!cache[i].busy &&
-            (victim < 0 || cache[i].last_use < cache[victim].last_use))
-          victim = (int)i;
-      if (victim < 0)
-        return false;
-      femtc_solver_destroy(cache[victim].solver);
-      used -= cache[victim].bytes;
-      cache.erase(cache.begin() + victim);
+      auto it = jobs.find(id);
+      if (it == jobs.end())
+        continue;
+      femtc_request_result res{};
+      res.tag = it->second.r.tag;
+      res.kind = it->second.r.kind;
+      res.status = FEMTC_ERR_INVALID_ARGUMENT;
+      res.slot = -1;
+      res.t_arrival = it->second.t_arrival;
+      res.t_start = res.t_end = now();
+      done.push_back(res);
+      jobs.erase(it);
     }
-    return true;
   }
 
   void worker(int slot)
@@ -130,61 +132,41 @@ struct femtc_server
     (void)hipSetDevice(device);
     hipStream_t st = nullptr;
     (void)hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    std::vector<void*> victims;
     for (;;)
     {
-      Pending job{};
-      femtc_solver* cached = nullptr; // resident solver taken by this job
-      size_t reserve = 0;             // budget reserved for a new solver
+      AdmTake t;
+      Job job{};
+      bool got = false;
       {
         std::unique_lock<std::mutex> lk(mu);
-        bool got = false;
         for (;;)
         {
-          if (stop && queue.empty())
-            break;
-          // first admissible request in arrival order
-          for (auto it = queue.begin(); it != queue.end(); ++it)
+          if (adm.take(slot, &t, &victims))
           {
-            if (it->r.kind == FEMTC_REQUEST_SOLVE)
-            {
-              const int c = find(it->r.problem, it->r.inner_dtype, slot);
-              if (c >= 0)
-              {
-                cached = cache[c].solver;
-                cache[c].busy = true;
-                cache[c].last_use = ++tick;
-              }
-              else
-              {
-                const size_t need =
-                    solver_bytes(*it->r.problem, it->r.inner_dtype);
-                if (!make_room(need))
-                  continue; // does not fit now: try a later request
-                used += need;
-                reserve = need;
-              }
-            }
-            job = *it;
-            queue.erase(it);
+            job = jobs.at(t.req.id);
+            ++running[job.key];
+            ++inflight;
             got = true;
             break;
           }
-          if (got)
+          if (stop && adm.queue_empty())
             break;
           cv_work.wait(lk);
         }
-        if (!got)
-          break;
-        ++inflight;
       }
+      destroy_all(victims); // evicted for this admission, outside the lock
+      if (!got)
+        break;
       femtc_request_result res{};
       res.tag = job.r.tag;
       res.kind = job.r.kind;
       res.slot = slot;
       res.t_arrival = job.t_arrival;
       res.t_start = now();
-      const femtc_problem& p = *job.r.problem;
-      femtc_solver* solver = nullptr;
+      const femtc_problem& p = job.p;
+      femtc_solver* solver = static_cast<femtc_solver*>(t.cached);
+      size_t actual = 0;
       if (job.r.kind == FEMTC_REQUEST_APPLY)
       {
         res.status = femtc_global_apply(p.op, FEMTC_F64, p.degree, p.nq,
@@ -196,10 +178,14 @@ struct femtc_server
       }
       else
       {
-        if (cached)
-          solver = cached;
-        else
+        if (!solver)
+        {
           res.status = femtc_solver_create(&solver, &p, job.r.inner_dtype, st);
+          if (res.status == FEMTC_OK)
+            (void)femtc_solver_bytes(solver, &actual);
+          else
+            solver = nullptr;
+        }
         if (res.status == FEMTC_OK)
         {
           femtc_solve_info info{};
@@ -212,26 +198,25 @@ struct femtc_server
         }
       }
       res.t_end = now();
+      void* dead = nullptr;
       {
         std::lock_guard<std::mutex> lk(mu);
         if (job.r.kind == FEMTC_REQUEST_SOLVE)
         {
-          if (cached)
-          {
-            // busy entries are never evicted, so it is still there
-            const int c = find(job.r.problem, job.r.inner_dtype, slot);
-            if (c >= 0)
-              cache[c].busy = false;
-          }
+          if (t.cached)
+            dead = adm.release(t, slot);
           else if (solver)
-            cache.push_back(Cached{job.r.problem, job.r.inner_dtype, slot,
-                                   solver, reserve, ++tick, false});
+            dead = adm.adopt(t, slot, solver, actual);
           else
-            used -= reserve; // creation failed
+            adm.cancel(t);
         }
+        jobs.erase(t.req.id);
+        --running[job.key];
         done.push_back(res);
         --inflight;
       }
+      if (dead) // its problem was unregistered meanwhile
+        femtc_solver_destroy(static_cast<femtc_solver*>(dead));
       cv_done.notify_all();
       cv_work.notify_all();
     }
@@ -248,7 +233,7 @@ int femtc_server_create(femtc_server** out, int slots, size_t hbm_budget)
     return fail(FEMTC_ERR_INVALID_ARGUMENT, "server needs 1..16 slots");
   femtc_server* s = new femtc_server;
   s->slots = slots;
-  s->budget = hbm_budget;
+  s->adm = Admission(hbm_budget);
   s->t0 = Clock::now();
   if (hipGetDevice(&s->device) != hipSuccess)
   {
@@ -261,6 +246,51 @@ int femtc_server_create(femtc_server** out, int slots, size_t hbm_budget)
   return FEMTC_OK;
 }
 
+int femtc_solver_workspace_bytes(const femtc_problem* p, int inner_dtype,
+                                 size_t* bytes)
+{
+  if (!p || !bytes)
+    return fail(FEMTC_ERR_INVALID_ARGUMENT, "NULL argument");
+  *bytes = solver_bytes(*p, inner_dtype);
+  return FEMTC_OK;
+}
+
+int femtc_server_register(femtc_server* s, const femtc_problem* p)
+{
+  if (!s || !p)
+    return fail(FEMTC_ERR_INVALID_ARGUMENT, "NULL argument");
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (s->reg.count(p))
+    return fail(FEMTC_ERR_INVALID_ARGUMENT, "problem is registered already");
+  s->reg[p] = femtc_server::Registered{s->adm.add_problem(), *p};
+  return FEMTC_OK;
+}
+
+int femtc_server_unregister(femtc_server* s, const femtc_problem* p)
+{
+  if (!s || !p)
+    return fail(FEMTC_ERR_INVALID_ARGUMENT, "NULL argument");
+  std::vector<void*> victims;
+  {
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->reg.find(p);
+    if (it == s->reg.end())
+      return fail(FEMTC_ERR_INVALID_ARGUMENT, "problem is not registered");
+    const uint64_t key = it->second.key;
+    s->reg.erase(it);
+    std::vector<int64_t> dropped;
+    s->adm.remove_problem(key, &victims, &dropped);
+    s->drop(dropped);
+    // its running requests finish (their solvers are destroyed by their
+    // slots) before the caller may free the problem's buffers
+    s->cv_done.wait(lk, [&] { return s->running[key] == 0; });
+    s->running.erase(key);
+  }
+  femtc_server::destroy_all(victims);
+  s->cv_done.notify_all();
+  return FEMTC_OK;
+}
+
 int femtc_server_submit(femtc_server* s, const femtc_request* r)
 {
   if (!s || !r || !r->problem || !r->input || !r->output)
@@ -269,18 +299,29 @@ int femtc_server_submit(femtc_server* s, const femtc_request* r)
     return fail(FEMTC_ERR_INVALID_ARGUMENT, "kind must be apply or solve");
   if (r->kind == FEMTC_REQUEST_SOLVE &&
       (!(r->rtol > 0.0) || r->maxit < 1 ||
-       (r->inner_dtype != FEMTC_F64 && r->inner_dtype != FEMTC_F32)))
+       (r->inner_dtype != FEMTC_F64 && r->inner_dtype != FEMTC_F32 &&
+        r->inner_dtype != FEMTC_BF16 && r->inner_dtype != FEMTC_BF16_ROUNDED)))
     return fail(FEMTC_ERR_INVALID_ARGUMENT, "bad solve parameters");
-  // a request that could never be admitted would wait forever
-  if (r->kind == FEMTC_REQUEST_SOLVE &&
-      solver_bytes(*r->problem, r->inner_dtype) > s->budget)
-    return fail(FEMTC_ERR_OUT_OF_MEMORY,
-                "the solve workspace exceeds the server's HBM budget");
   {
     std::lock_guard<std::mutex> lk(s->mu);
     if (s->stop)
       return fail(FEMTC_ERR_INVALID_ARGUMENT, "server is stopping");
-    s->queue.push_back(femtc_server::Pending{*r, s->now()});
+    auto it = s->reg.find(r->problem);
+    if (it == s->reg.end())
+      return fail(FEMTC_ERR_INVALID_ARGUMENT,
+                  "the request's problem is not registered "
+                  "(femtc_server_register)");
+    const size_t bytes = r->kind == FEMTC_REQUEST_SOLVE
+                             ? solver_bytes(it->second.copy, r->inner_dtype)
+                             : 0;
+    // a request that could never be admitted would wait forever
+    if (bytes > s->adm.budget())
+      return fail(FEMTC_ERR_OUT_OF_MEMORY,
+                  "the solve workspace exceeds the server's HBM budget");
+    const int64_t id = ++s->next_id;
+    const double t = s->now();
+    s->jobs[id] = femtc_server::Job{*r, it->second.copy, it->second.key, t};
+    s->adm.push(AdmReq{id, r->kind, it->second.key, r->inner_dtype, bytes, t});
   }
   s->cv_work.notify_all();
   return FEMTC_OK;
@@ -291,7 +332,8 @@ int femtc_server_drain(femtc_server* s)
   if (!s)
     return fail(FEMTC_ERR_INVALID_ARGUMENT, "server is NULL");
   std::unique_lock<std::mutex> lk(s->mu);
-  s->cv_done.wait(lk, [s] { return s->queue.empty() && s->inflight == 0; });
+  s->cv_done.wait(lk,
+                  [s] { return s->adm.queue_empty() && s->inflight == 0; });
   return FEMTC_OK;
 }
 
@@ -315,9 +357,9 @@ int femtc_server_memory(const femtc_server* s, size_t* used, int* cached)
     return fail(FEMTC_ERR_INVALID_ARGUMENT, "server is NULL");
   std::lock_guard<std::mutex> lk(const_cast<femtc_server*>(s)->mu);
   if (used)
-    *used = s->used;
+    *used = s->adm.used();
   if (cached)
-    *cached = (int)s->cache.size();
+    *cached = s->adm.resident();
   return FEMTC_OK;
 }
 
@@ -332,8 +374,9 @@ int femtc_server_destroy(femtc_server* s)
   s->cv_work.notify_all();
   for (auto& t : s->workers)
     t.join();
-  for (auto& c : s->cache)
-    femtc_solver_destroy(c.solver);
+  std::vector<void*> victims;
+  s->adm.drain(&victims);
+  femtc_server::destroy_all(victims);
   delete s;
   return FEMTC_OK;
 }

@@ -661,6 +661,8 @@ struct femtc_solver
   bool graph_failed = false;
   hipStream_t cap_stream = nullptr;
   hipGraphExec_t graph = nullptr;
+  // device bytes allocated for the workspace (femtc_solver_bytes)
+  size_t bytes = 0;
 };
 
 namespace femtc
@@ -668,7 +670,8 @@ namespace femtc
 namespace
 {
 
-int dev_alloc(void** p, size_t bytes)
+// Device allocation of solver S, counted in S->bytes.
+int dev_alloc(femtc_solver* S, void** p, size_t bytes)
 {
   if (bytes == 0)
     bytes = 8;
@@ -680,6 +683,7 @@ int dev_alloc(void** p, size_t bytes)
                 "solver workspace allocation failed (" +
                     std::to_string(bytes) + " bytes)");
   }
+  S->bytes += bytes;
   return FEMTC_OK;
 }
 
@@ -934,7 +938,7 @@ int choose_variant(femtc_solver* S, int structured)
     S->inner_assembled = true;
     S->even_odd = false;
     S->nepart = S->prob.num_cells; // one energy partial per cell (wave)
-    return dev_alloc(&S->lattice,
+    return dev_alloc(S, &S->lattice,
                      4 * (size_t)lattice_size(S->prob.grid[0], S->prob.grid[1],
                                               S->prob.grid[2], S->N - 1));
   }
@@ -944,7 +948,7 @@ int choose_variant(femtc_solver* S, int structured)
   if (try_asm)
   {
     const size_t es = S->inner_dtype == FEMTC_F64 ? 8 : 4;
-    if (dev_alloc(&S->lattice,
+    if (dev_alloc(S, &S->lattice,
                   es * (size_t)lattice_size(S->prob.grid[0], S->prob.grid[1],
                                             S->prob.grid[2], S->N - 1)))
     {
@@ -1001,6 +1005,9 @@ int choose_variant(femtc_solver* S, int structured)
   {
     dev_free(S->lattice);
     S->lattice = nullptr;
+    S->bytes -= (S->inner_dtype == FEMTC_F64 ? 8 : 4) *
+                (size_t)lattice_size(S->prob.grid[0], S->prob.grid[1],
+                                     S->prob.grid[2], S->N - 1);
   }
   return st;
 }
@@ -1272,23 +1279,23 @@ int femtc_solver_create(femtc_solver** out, const femtc_problem* prob,
   hipStream_t s = S->stream;
   do
   {
-    if ((st = dev_alloc((void**)&S->diag, 8 * n)) ||
-        (st = dev_alloc((void**)&S->dinv64, 8 * n)) ||
-        (st = dev_alloc((void**)&S->r64, 8 * n)) ||
-        (st = dev_alloc((void**)&S->Au64, 8 * n)) ||
-        (st = dev_alloc((void**)&S->part, 8 * 2 * NBLK)) ||
-        (st = dev_alloc((void**)&S->dscal, 8)) ||
-        (st = dev_alloc((void**)&S->dbad, sizeof(int))) ||
-        (st = dev_alloc((void**)&S->st, sizeof(PcgState))) ||
-        (st = dev_alloc(&S->rhs, vs * n)) || (st = dev_alloc(&S->e, vs * n)) ||
-        (st = dev_alloc(&S->p, vs * n)) || (st = dev_alloc(&S->Ap, vs * n)))
+    if ((st = dev_alloc(S, (void**)&S->diag, 8 * n)) ||
+        (st = dev_alloc(S, (void**)&S->dinv64, 8 * n)) ||
+        (st = dev_alloc(S, (void**)&S->r64, 8 * n)) ||
+        (st = dev_alloc(S, (void**)&S->Au64, 8 * n)) ||
+        (st = dev_alloc(S, (void**)&S->part, 8 * 2 * NBLK)) ||
+        (st = dev_alloc(S, (void**)&S->dscal, 8)) ||
+        (st = dev_alloc(S, (void**)&S->dbad, sizeof(int))) ||
+        (st = dev_alloc(S, (void**)&S->st, sizeof(PcgState))) ||
+        (st = dev_alloc(S, &S->rhs, vs * n)) || (st = dev_alloc(S, &S->e, vs * n)) ||
+        (st = dev_alloc(S, &S->p, vs * n)) || (st = dev_alloc(S, &S->Ap, vs * n)))
       break;
     // one energy partial per wave of the operator launch, whose cell
     // groups depend on the scatter (set again once it is chosen)
     S->nepart = bf ? C
                    : std::max(energy_partials(C, S->Q, false, inner_dtype),
                               energy_partials(C, S->Q, true, inner_dtype));
-    if ((st = dev_alloc((void**)&S->epart, 8 * S->nepart)))
+    if ((st = dev_alloc(S, (void**)&S->epart, 8 * S->nepart)))
       break;
     if (hipHostMalloc((void**)&S->hst, sizeof(PcgState)) != hipSuccess ||
         hipHostMalloc((void**)&S->hscal, sizeof(double)) != hipSuccess)
@@ -1305,8 +1312,8 @@ int femtc_solver_create(femtc_solver** out, const femtc_problem* prob,
     const int64_t ntab = 2 * (int64_t)S->N * S->Q;
     if (aff)
     {
-      if ((st = dev_alloc((void**)&S->tabw64, 8 * (ntab + S->Q))) ||
-          (st = dev_alloc((void**)&S->cg64, 8 * CELLGEO * C)))
+      if ((st = dev_alloc(S, (void**)&S->tabw64, 8 * (ntab + S->Q))) ||
+          (st = dev_alloc(S, (void**)&S->cg64, 8 * CELLGEO * C)))
         break;
       std::vector<double> xq(S->Q), wq(S->Q);
       femtc_gauss_rule(S->Q, xq.data(), wq.data());
@@ -1320,6 +1327,21 @@ int femtc_solver_create(femtc_solver** out, const femtc_problem* prob,
       }
       if ((st = femtc_geometry_cell(C, prob->J, S->cg64, FEMTC_F64, s)))
         break;
+    }
+    // every dofmap entry in range before the first kernel scatters
+    // through it
+    {
+      int inrange = 0;
+      if ((st = check_dofmap_range(prob->dofmap,
+                                   C * S->N * S->N * S->N, n, s,
+                                   &inrange)) != FEMTC_OK)
+        break;
+      if (!inrange)
+      {
+        st = fail(FEMTC_ERR_INVALID_ARGUMENT,
+                  "a dofmap entry is outside [0, ndofs)");
+        break;
+      }
     }
     // Jacobi preconditioner from the FP64 operator
     if (S->outer_affine)
@@ -1346,10 +1368,10 @@ int femtc_solver_create(femtc_solver** out, const femtc_problem* prob,
     else if (aff)
     {
       // FP32 inner operator on affine cells: 7 values per cell
-      if ((st = dev_alloc(&S->r, 4 * n)) || (st = dev_alloc(&S->dinv, 2 * n)) ||
-          (st = dev_alloc(&S->dinv_v, 4 * n)) ||
-          (st = dev_alloc(&S->tab, 4 * (ntab + S->Q))) ||
-          (st = dev_alloc(&S->G, 4 * CELLGEO * C)))
+      if ((st = dev_alloc(S, &S->r, 4 * n)) || (st = dev_alloc(S, &S->dinv, 2 * n)) ||
+          (st = dev_alloc(S, &S->dinv_v, 4 * n)) ||
+          (st = dev_alloc(S, &S->tab, 4 * (ntab + S->Q))) ||
+          (st = dev_alloc(S, &S->G, 4 * CELLGEO * C)))
         break;
       hipLaunchKernelGGL(k_precond_bf16, dim3(NBLK), dim3(RB), 0, s,
                          (const double*)S->dinv64, (uint16_t*)S->dinv,
@@ -1363,9 +1385,9 @@ int femtc_solver_create(femtc_solver** out, const femtc_problem* prob,
     }
     else
     {
-      if ((st = dev_alloc(&S->r, 4 * n)) || (st = dev_alloc(&S->dinv, 2 * n)) ||
-          (st = dev_alloc(&S->dinv_v, 4 * n)) ||
-          (st = dev_alloc(&S->tab, 4 * 2 * S->N * S->Q)))
+      if ((st = dev_alloc(S, &S->r, 4 * n)) || (st = dev_alloc(S, &S->dinv, 2 * n)) ||
+          (st = dev_alloc(S, &S->dinv_v, 4 * n)) ||
+          (st = dev_alloc(S, &S->tab, 4 * 2 * S->N * S->Q)))
         break;
       hipLaunchKernelGGL(k_precond_bf16, dim3(NBLK), dim3(RB), 0, s,
                          (const double*)S->dinv64, (uint16_t*)S->dinv,
@@ -1375,14 +1397,14 @@ int femtc_solver_create(femtc_solver** out, const femtc_problem* prob,
                          (int64_t)2 * S->N * S->Q);
       if (prob->G)
       {
-        if ((st = dev_alloc(&S->G, 4 * 6 * Q3 * C)))
+        if ((st = dev_alloc(S, &S->G, 4 * 6 * Q3 * C)))
           break;
         hipLaunchKernelGGL((k_convert<double, float>), dim3(NBLK), dim3(RB),
                            0, s, prob->G, (float*)S->G, 6 * Q3 * C);
       }
       if (prob->detJw)
       {
-        if ((st = dev_alloc(&S->detJw, 4 * Q3 * C)))
+        if ((st = dev_alloc(S, &S->detJw, 4 * Q3 * C)))
           break;
         hipLaunchKernelGGL((k_convert<double, float>), dim3(NBLK), dim3(RB),
                            0, s, prob->detJw, (float*)S->detJw, Q3 * C);
@@ -1395,7 +1417,7 @@ int femtc_solver_create(femtc_solver** out, const femtc_problem* prob,
       size_t ob = 0;
       if ((st = femtc_mfma_tables_bytes(S->mfma_split, &ob)) ||
           (st = sync_read(S, th.data(), prob->tables, 8 * ntab)) ||
-          (st = dev_alloc(&S->mfma_ops, ob)) ||
+          (st = dev_alloc(S, &S->mfma_ops, ob)) ||
           (st = femtc_mfma_tables(prob->degree, prob->nq, S->mfma_split,
                                   th.data(), S->mfma_ops)))
         break;
@@ -1462,6 +1484,14 @@ int femtc_solver_create(femtc_solver** out, const femtc_problem* prob,
 int femtc_solver_destroy(femtc_solver* solver)
 {
   destroy(solver);
+  return FEMTC_OK;
+}
+
+int femtc_solver_bytes(const femtc_solver* solver, size_t* bytes)
+{
+  if (!solver || !bytes)
+    return fail(FEMTC_ERR_INVALID_ARGUMENT, "NULL argument");
+  *bytes = solver->bytes;
   return FEMTC_OK;
 }
 

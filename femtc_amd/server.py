@@ -39,10 +39,22 @@ class ResidentProblem:
                    it.get("dims") if structured else None)
 
 
+def workspace_bytes(problem: ResidentProblem, inner: int = _lib.F32) -> int:
+    """The device bytes the server reserves for a new solver of problem."""
+    n = C.c_size_t()
+    _lib.check(_lib.lib().femtc_solver_workspace_bytes(
+        C.byref(problem.struct), inner, C.byref(n)), "solver_workspace_bytes")
+    return n.value
+
+
 class Server:
     """The slots run on streams of their own: a request's input must be
     complete on the device (e.g. torch.cuda.synchronize()) when it is
-    submitted, and its output is ready once its result is returned."""
+    submitted, and its output is ready once its result is returned.
+
+    A problem is registered with the server at its first request (or by
+    register()) and stays registered, and alive, until unregister() or
+    close(); unregistering destroys its resident solvers."""
 
     def __init__(self, slots: int = 4, hbm_budget: int = 8 << 30):
         h = C.c_void_p()
@@ -51,8 +63,26 @@ class Server:
                    "server_create")
         self._h = h
         self._inflight = {}  # tag -> tensors kept alive until completion
+        self._problems = {}  # id -> registered ResidentProblem
+
+    def register(self, problem: ResidentProblem) -> None:
+        if id(problem) in self._problems:
+            return
+        _lib.check(_lib.lib().femtc_server_register(
+            self._h, C.byref(problem.struct)), "server_register")
+        self._problems[id(problem)] = problem
+
+    def unregister(self, problem: ResidentProblem) -> None:
+        """Forget the problem: its queued requests complete with an error,
+        its resident solvers are destroyed (after running ones finish)."""
+        if id(problem) not in self._problems:
+            return
+        _lib.check(_lib.lib().femtc_server_unregister(
+            self._h, C.byref(problem.struct)), "server_unregister")
+        del self._problems[id(problem)]
 
     def _submit(self, req, keep):
+        self.register(keep[0])
         self._inflight[req.tag] = keep
         st = _lib.lib().femtc_server_submit(self._h, C.byref(req))
         if st != _lib.OK:
@@ -103,6 +133,7 @@ class Server:
             _lib.lib().femtc_server_destroy(self._h)
             self._h = None
             self._inflight.clear()
+            self._problems.clear()
 
     def __del__(self):
         try:
@@ -135,3 +166,31 @@ def latency_stats(results) -> dict:
         out["solve_p50_ms"] = float(np.percentile(lat[is_solve], 50))
         out["solve_p99_ms"] = float(np.percentile(lat[is_solve], 99))
     return out
+
+
+def simulate_admission(requests, slots: int, budget: int, nproblems=None,
+                       remove_at=None):
+    """Run the server's admission, queueing and eviction code with a fake
+    clock and a fake allocator (no device work).  requests: dicts with
+    t_arrival, kind (REQUEST_APPLY / REQUEST_SOLVE), problem, inner, bytes,
+    actual, service, create.  Returns (results, stats) as dicts."""
+    n = len(requests)
+    if nproblems is None:
+        nproblems = 1 + max((r["problem"] for r in requests), default=0)
+    reqs = (_lib.SimRequest * max(n, 1))()
+    for i, r in enumerate(requests):
+        reqs[i] = _lib.SimRequest(r["t_arrival"], r["kind"], r["problem"],
+                                  r.get("inner", _lib.F32), r.get("bytes", 0),
+                                  r.get("actual", r.get("bytes", 0)),
+                                  r["service"], r.get("create", 0.0))
+    out = (_lib.SimResult * max(n, 1))()
+    stats = _lib.SimStats()
+    rem = None
+    if remove_at is not None:
+        rem = (C.c_double * nproblems)(*remove_at)
+    _lib.check(_lib.lib().femtc_admission_simulate(
+        reqs, n, nproblems, rem, slots, budget, out, C.byref(stats)),
+        "admission_simulate")
+    res = [{k: getattr(out[i], k) for k, _ in _lib.SimResult._fields_}
+           for i in range(n)]
+    return res, {k: getattr(stats, k) for k, _ in _lib.SimStats._fields_}

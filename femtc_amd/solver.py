@@ -77,6 +77,13 @@ class PCGSolver:
                    "solver_scatter")
         return a.value
 
+    def bytes(self) -> int:
+        """Device bytes the solver holds."""
+        n = C.c_size_t()
+        _lib.check(_lib.lib().femtc_solver_bytes(self._h, C.byref(n)),
+                   "solver_bytes")
+        return n.value
+
     def even_odd(self) -> bool:
         """Whether the solver's operators contract in even-odd form."""
         a = C.c_int32()

@@ -502,6 +502,12 @@ typedef struct femtc_solver femtc_solver; /* opaque */
  * not positive (a dof no cell touches). */
 int femtc_solver_create(femtc_solver** solver, const femtc_problem* problem,
                         int inner_dtype, void* stream);
+/* Device bytes the solver holds (its workspace, tables and graphs aside). */
+int femtc_solver_bytes(const femtc_solver* solver, size_t* bytes);
+/* The estimate of a solver's device bytes that the request server reserves
+ * while the solver is created (host only, no device work). */
+int femtc_solver_workspace_bytes(const femtc_problem* problem,
+                                 int inner_dtype, size_t* bytes);
 /* Solve A u = f (f, u FP64 device vectors of ndofs; u overwritten) to
  * ||f - A u|| <= rtol ||f||.  Plain PCG stops on its recursive residual;
  * defect correction stops on the FP64 residual recomputed after each inner
@@ -546,7 +552,8 @@ enum
 typedef struct femtc_request
 {
   int kind;
-  const femtc_problem* problem; /* must outlive the request */
+  /* a problem registered with femtc_server_register (its copy is used) */
+  const femtc_problem* problem;
   const void* input;            /* FP64 device vector of ndofs */
   void* output;                 /* FP64 device vector of ndofs */
   double rtol;                  /* solve */
@@ -570,8 +577,20 @@ typedef struct femtc_request_result
 
 typedef struct femtc_server femtc_server; /* opaque */
 int femtc_server_create(femtc_server** server, int slots, size_t hbm_budget);
-/* Queue a request (non-blocking); FEMTC_ERR_OUT_OF_MEMORY if its solve
- * workspace alone exceeds the budget. */
+/* Make a problem known to the server, which keeps a copy of the struct; its
+ * device buffers must stay alive and unchanged until it is unregistered.
+ * Requests name it by the pointer registered.  Registering a pointer that
+ * is registered already fails. */
+int femtc_server_register(femtc_server* server, const femtc_problem* problem);
+/* Forget a problem: its queued requests complete with
+ * FEMTC_ERR_INVALID_ARGUMENT, its idle resident solvers are destroyed now
+ * and busy ones when their request ends, so the address may then be reused
+ * for another problem without meeting a stale solver. */
+int femtc_server_unregister(femtc_server* server,
+                            const femtc_problem* problem);
+/* Queue a request (non-blocking); FEMTC_ERR_INVALID_ARGUMENT for an
+ * unregistered problem, FEMTC_ERR_OUT_OF_MEMORY if its solve workspace
+ * alone exceeds the budget. */
 int femtc_server_submit(femtc_server* server, const femtc_request* request);
 /* Block until every submitted request has completed. */
 int femtc_server_drain(femtc_server* server);
@@ -582,6 +601,50 @@ int femtc_server_memory(const femtc_server* server, size_t* used,
                         int* cached_solvers);
 /* Finish the queued requests, then stop the slots and free the solvers. */
 int femtc_server_destroy(femtc_server* server);
+
+/* ---------------------------------------------------------------------- */
+/* The server's admission, queueing and eviction code driven by a fake     */
+/* clock and a fake allocator (host only, no device work; no counterpart   */
+/* in the reference), for tests of the serving logic.                      */
+/* ---------------------------------------------------------------------- */
+typedef struct femtc_sim_request
+{
+  double t_arrival; /* seconds */
+  int kind;         /* FEMTC_REQUEST_APPLY or FEMTC_REQUEST_SOLVE */
+  int problem;      /* 0 .. nproblems-1 */
+  int inner;        /* inner precision (part of the workspace key) */
+  size_t bytes;     /* a solve's workspace estimate (charged at admission) */
+  size_t actual;    /* its size once created (charged from then on) */
+  double service;   /* run time with a resident workspace */
+  double create;    /* extra run time when the workspace is created */
+} femtc_sim_request;
+
+typedef struct femtc_sim_result
+{
+  int status; /* FEMTC_OK; FEMTC_ERR_OUT_OF_MEMORY: refused at submission;
+                 FEMTC_ERR_INVALID_ARGUMENT: its problem was removed first */
+  int slot;
+  int cached; /* 1 when a resident workspace served it */
+  double t_start, t_end;
+} femtc_sim_result;
+
+typedef struct femtc_sim_stats
+{
+  size_t high_water;     /* largest charge against the budget */
+  int64_t evictions;     /* idle workspaces evicted for room */
+  int64_t created;       /* workspaces created */
+  int64_t destroyed;     /* workspaces destroyed (evicted or retired) */
+  double makespan;       /* time of the last event */
+  int64_t left_in_queue; /* requests never admitted */
+} femtc_sim_stats;
+
+/* Run the requests (any order of arrival times) through `slots` slots and
+ * a budget of `budget` bytes.  remove_at[p] >= 0 removes problem p at that
+ * time (NULL: never).  out[i] is the result of reqs[i]. */
+int femtc_admission_simulate(const femtc_sim_request* reqs, int n,
+                             int nproblems, const double* remove_at, int slots,
+                             size_t budget, femtc_sim_result* out,
+                             femtc_sim_stats* stats);
 
 #ifdef __cplusplus
 }
