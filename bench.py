@@ -169,7 +169,7 @@ def structured(torch, F, n, P, want_detJw=False):
     dm = torch.empty(C * N ** 3, device=dev, dtype=torch.int32)
     F.dofmap_structured(n, n, n, P, dm)
     return {"tab": tab, "G": G, "D": D, "dm": dm, "C": C, "ng": ng, "P": P,
-            "Q": Q, "N": N, "J": dJ}
+            "Q": Q, "N": N, "J": dJ, "n": n}
 
 
 # Applies run at setup, before the warm-up steps.  On a freshly started GPU
@@ -212,26 +212,49 @@ def time_apply(torch, F, pr, steps, warmup, x, y, settle=SETTLE_APPLIES):
     return wall * 1e3 / steps, kern_ms
 
 
-def copy_bandwidth(torch, mib=2048, reps=10):
-    """Device-to-device copy rate (read + write bytes / time) of a 2 GiB
-    buffer: the practical HBM ceiling beside the 8 TB/s peak."""
-    m = mib * (1 << 20) // 8
-    src = torch.ones(m, device="cuda", dtype=torch.float64)
-    dst = torch.empty_like(src)
-    for _ in range(3):
-        dst.copy_(src)
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(reps):
-        dst.copy_(src)
-    e1.record()
-    torch.cuda.synchronize()
-    sec = e0.elapsed_time(e1) * 1e-3 / reps
-    del src, dst
+def stream_bandwidth(torch, F, n=64 << 20, reps=10):
+    """The practical HBM ceiling from the library's hand-written streaming
+    probe (femtc_stream_probe, 16-byte non-temporal loads), on arrays of n
+    doubles (512 MiB each), mean over reps after two warm-up launches, the
+    best over grids of 1, 2, 4 and 8 workgroups per CU:
+      read5_write1: five arrays read, one written (the apply's mix, 3 GiB)
+      copy:         one read, one written (1 GiB)
+      read_only:    six arrays read (3 GiB)"""
+    from femtc_amd import _lib
+    src = torch.ones(6 * n, device="cuda", dtype=torch.float64)
+    dst = torch.empty(n, device="cuda", dtype=torch.float64)
+    sink = torch.empty(1 << 16, device="cuda", dtype=torch.float64)
+    stream = torch.cuda.current_stream()
+    L = _lib.lib()
+    out = {}
+    for name, nr, nw in (("read5_write1", 5, 1), ("copy", 1, 1),
+                         ("read_only", 6, 0)):
+        best, best_wg = 0.0, 0
+        for wg in (1, 2, 4, 8):
+            def go():
+                _lib.check(L.femtc_stream_probe(nr, nw, src.data_ptr(),
+                                                dst.data_ptr(), n,
+                                                sink.data_ptr(), sink.numel(),
+                                                wg, stream.cuda_stream),
+                           "stream_probe")
+            go()
+            go()
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(reps):
+                go()
+            e1.record(stream)
+            torch.cuda.synchronize()
+            sec = e0.elapsed_time(e1) * 1e-3 / reps
+            gbs = 8 * n * (nr + nw) / sec / 1e9
+            if gbs > best:
+                best, best_wg = gbs, wg
+        out[name + "_GBs"] = best
+        out[name + "_wg_per_cu"] = best_wg
+    del src, dst, sink
     torch.cuda.empty_cache()
-    return 2 * 8 * m / sec / 1e9
-
+    return out
 
 def alg_bytes_stiffness(pr):
     return 8 * pr["ng"] * 2 + 4 * pr["C"] * pr["N"] ** 3 + 48 * pr["C"] * pr["Q"] ** 3
@@ -358,41 +381,59 @@ def bench_mfma_apply(torch, F, n=48, P=6, reps=20):
     return out
 
 
+def plan_scatters(torch, F, pr, x, y, steps=30, warmup=60):
+    """The apply through a plan on the structured dofmap: every scatter
+    timed on its own (ms per overwriting apply, clearing included), then the
+    autotuned choice."""
+    n, P, Q = pr["n"], pr["P"], pr["Q"]
+    ab = alg_bytes_stiffness(pr)
+    plan = F.Plan(n, n, n, P, pr["dm"], pr["ng"])
+    stream = torch.cuda.current_stream()
+    names = {(0, 0, 0): "atomic", (-1, -1, -1): "assembled",
+             (-2, -2, -2): "structured atomic",
+             (-3, -3, -3): "structured z-run"}
+
+    def timed():
+        for _ in range(warmup):
+            plan.apply(F.OP_STIFFNESS, pr["tab"], x, y, Q, G=pr["G"])
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(steps):
+            plan.apply(F.OP_STIFFNESS, pr["tab"], x, y, Q, G=pr["G"])
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / steps
+
+    by_scatter = {}
+    for shape, name in names.items():
+        plan.set_block(Q, F.F64, F.OP_STIFFNESS, shape)
+        t = timed()
+        by_scatter[name] = {"ms_per_apply": t, "roofline_frac":
+                            ab / (t * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    shape, _ = plan.autotune(F.OP_STIFFNESS, pr["tab"], x, y, Q, G=pr["G"],
+                             reps=5)
+    pms = timed()
+    plan.close()
+    return {"scatter": names.get(tuple(shape), f"blocked {tuple(shape)}"),
+            "ms_per_apply": pms,
+            "gdofs": pr["ng"] / (pms * 1e-3) / 1e9,
+            "roofline_frac": ab / (pms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "by_scatter": by_scatter}
+
+
 def bench_apply_p4(torch, F, steps=30, warmup=60):
     pr = structured(torch, F, 64, 4)
     x = torch.from_numpy(_uniform(4, pr["ng"], -1, 1)).cuda()
     y = torch.empty(pr["ng"], device="cuda", dtype=torch.float64)
     ms, kern = time_apply(torch, F, pr, steps, warmup, x, y)
     ab = alg_bytes_stiffness(pr)
-    # the same apply through an autotuned plan (direct, blocked or assembled
-    # scatter, whichever times fastest on these buffers)
-    plan = F.Plan(64, 64, 64, 4, pr["dm"], pr["ng"])
-    shape, _ = plan.autotune(F.OP_STIFFNESS, pr["tab"], x, y, 5, G=pr["G"],
-                             reps=5)
-    stream = torch.cuda.current_stream()
-    for _ in range(warmup):
-        plan.apply(F.OP_STIFFNESS, pr["tab"], x, y, 5, G=pr["G"])
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    for _ in range(steps):
-        plan.apply(F.OP_STIFFNESS, pr["tab"], x, y, 5, G=pr["G"])
-    e1.record(stream)
-    torch.cuda.synchronize()
-    pms = e0.elapsed_time(e1) / steps
-    plan.close()
     return {"workload": "stiffness apply y=Kx, 64^3 mesh, P=4 GLL, Q=5, "
                         "int32 dofmap, FP64",
             "gdofs": pr["ng"] / (ms * 1e-3) / 1e9, "kernel_ms": kern,
             "algorithmic_bytes": ab,
             "roofline_frac": ab / (kern * 1e-3) / 1e9 / HBM_PEAK_GBS,
-            "plan": {"scatter": {(0, 0, 0): "atomic",
-                                 (-1, -1, -1): "assembled"}.get(
-                                     tuple(shape), f"blocked {tuple(shape)}"),
-                     "ms_per_apply": pms,
-                     "gdofs": pr["ng"] / (pms * 1e-3) / 1e9,
-                     "roofline_frac": ab / (pms * 1e-3) / 1e9 / HBM_PEAK_GBS}}
-
+            "plan": plan_scatters(torch, F, pr, x, y, steps, warmup)}
 
 def bench_high_degree(torch, F, reps=10):
     """Cell-local mass and stiffness applies at degrees 9..25 (the
@@ -573,11 +614,16 @@ def main():
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     if a.copy_calibration:
+        # the hand-written copy (femtc_stream_probe, one read, one write) of
+        # a.copy_calibration MiB, three times: the counter calibration run
+        from femtc_amd import _lib
         m = a.copy_calibration * (1 << 20) // 8
         src = torch.ones(m, device=dev, dtype=torch.float64)
         dst = torch.empty_like(src)
         for _ in range(3):
-            dst.copy_(src)
+            _lib.check(_lib.lib().femtc_stream_probe(
+                1, 1, src.data_ptr(), dst.data_ptr(), m, None, 0, 4,
+                torch.cuda.current_stream().cuda_stream), "stream_probe")
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes": 8 * m}), flush=True)
         return
@@ -631,6 +677,10 @@ def main():
         "cell_local_gdofs": C * N ** 3 / (ms_per_step * 1e-3) / 1e9,
         "check_sum_y": s,
     }
+    if not a.apply_only:
+        # the same apply through a plan (dofmap verified structured once at
+        # setup), every scatter and the autotuned pick
+        out["plan"] = plan_scatters(torch, F, pr, x, y)
     del x, y, pr
     torch.cuda.empty_cache()
     if not a.apply_only:
@@ -650,8 +700,10 @@ def main():
         if "solve" in out:
             out["solve_speedup_refined_vs_fp64"] = out["solve"][
                 "speedup_refined_vs_fp64"]
-        if not a.only:
-            out["hbm_copy_GBs"] = copy_bandwidth(torch)
+        if not a.only or a.only == "apply_p4_64cube":
+            st = stream_bandwidth(torch, F)
+            out["hbm_stream"] = st
+            out["hbm_copy_GBs"] = st["copy_GBs"]
     if not a.no_cpu_baseline:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(P, C, ng)

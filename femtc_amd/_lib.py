@@ -61,6 +61,8 @@ SIGNATURES = {
     "femtc_global_apply_add": (_i32, [_i32, _i32, _i32, _i32, _vp, _vp, _vp,
                                       _i64, _vp, _vp, _vp, _i64, _vp]),
     "femtc_fill_zero": (_i32, [_vp, _i64, _i32, _vp]),
+    "femtc_stream_probe": (_i32, [_i32, _i32, _vp, _vp, _i64, _vp, _i32,
+                                  _i32, _vp]),
     "femtc_mass_operator": (_i32, [_i32, _vp, _sz, _vp, _vp, _vp, _sz, _i32,
                                    _i32, _i32, _i32, _i32]),
     "femtc_stiffness_operator": (_i32, [_i32, _vp, _sz, _vp, _vp, _vp, _i32,

@@ -106,6 +106,14 @@ __device__ inline void atomic_add(double* p, double v)
 }
 __device__ inline void atomic_add(float* p, float v) { unsafeAtomicAdd(p, v); }
 
+// Atomic add into LDS (ds_add_f64 / ds_add_f32): lanes of one instruction
+// that hit the same address all count.
+template <typename T>
+__device__ inline void lds_add(T* p, T v)
+{
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 template <typename T>
 __device__ inline T stream_load(const T* p)
 {
@@ -339,6 +347,41 @@ inline size_t blocked_box_bytes(size_t box)
   return box * (sizeof(T) + 2 * sizeof(int32_t));
 }
 
+// Run length the z-run scatter aims for: consecutive global dofs along z
+// per atomic run.  Measured on MI355X, FP64 atomics at the memory side
+// sustain 61 G/s in runs of 5 consecutive addresses and 141 G/s in runs of
+// 33.
+constexpr int ZRUN_TARGET = 32;
+// LDS of a z-run workgroup at most: four workgroups per CU.  At P=4 (FP64
+// stiffness, 64^3) runs of 17, 25 and 33 dofs took 396, 392 and 381 us per
+// apply; at P=6 (48^3) runs of 19 took 497 us and the 31-dof segments of a
+// 64 KiB workgroup (two per CU) 915 us.
+constexpr size_t ZRUN_LDS = 40 * 1024;
+
+// Waves per workgroup of the z-run scatter, each taking CPB cells of a
+// segment along z: enough that a segment's run (cells * P + 1 dofs) spans
+// ZRUN_TARGET values, as long as the waves' work areas and the segment's
+// dof box fit ZRUN_LDS.
+template <typename T, int N, int Q, int OP>
+constexpr int zrun_waves()
+{
+  constexpr int CPB = cells_per_block<Q>();
+  constexpr int want = (ZRUN_TARGET + (N - 1) - 1) / (N - 1); // cells
+  int w = (want + CPB - 1) / CPB;
+  while (w > 1 && w * blocked_work_bytes<T, N, Q, OP>() +
+                          sizeof(T) * N * N * ((size_t)w * CPB * (N - 1) + 1) >
+                      ZRUN_LDS)
+    --w;
+  return w;
+}
+
+// Cells per z-segment of the z-run scatter.
+template <typename T, int N, int Q, int OP>
+constexpr int zrun_cells()
+{
+  return zrun_waves<T, N, Q, OP>() * cells_per_block<Q>();
+}
+
 // The work of one workgroup (block index blk).
 // BW = 0: CPB consecutive cells, direct scatter.
 // BW > 0: blocked scatter, BW waves per workgroup and one block of
@@ -355,14 +398,21 @@ apply_body(const T* __restrict__ tab, const T* __restrict__ x,
            T* __restrict__ y, const T* __restrict__ G,
            const T* __restrict__ detJw, const int32_t* __restrict__ dofmap,
            long long num_cells, const BlockArgs& ba, long long blk,
-           double* __restrict__ epart = nullptr)
+           double* __restrict__ epart = nullptr, T* __restrict__ zbox = nullptr,
+           long long zcell0 = 0, int zncell = 0)
 {
   constexpr bool BLK = BW > 0;
   constexpr bool ASM = BW == SCATTER_ASSEMBLED;
-  static_assert(!BLK || (GATHER && Q * Q <= 64), "blocked scatter layout");
-  static_assert(!ASM || (GATHER && !REF), "assembled scatter");
+  constexpr bool STR = BW == SCATTER_STRUCT;
+  constexpr bool ZR = BW == SCATTER_ZRUN;
+  constexpr bool CLOSED = ASM || STR || ZR; // gather indices in closed form
+  // multi-wave workgroups whose waves own whole cells and their own LDS
+  // work area (blocked and z-run scatters)
+  constexpr bool WL = BLK || ZR;
+  static_assert(!WL || (GATHER && Q * Q <= 64), "blocked scatter layout");
+  static_assert(!CLOSED || (GATHER && !REF), "structured scatters");
   constexpr int CPB = cells_per_group<T, Q, BW>();
-  constexpr int BT = BLK ? 64 : group_threads<T, Q, BW>(); // threads per group
+  constexpr int BT = WL ? 64 : group_threads<T, Q, BW>(); // threads per group
   constexpr int NL = Q * Q;
   constexpr bool AFF = OP >= OP_MASS_AFF; // geometry from 7 values per cell
   constexpr int OPB = AFF ? OP - OP_MASS_AFF : OP;
@@ -371,8 +421,8 @@ apply_body(const T* __restrict__ tab, const T* __restrict__ x,
   constexpr bool GRAD = (OPB != OP_MASS);
   constexpr bool FULL = (OPB == OP_STIFF || OPB == OP_KPM); // 3 gradients
   constexpr bool GEO = GRAD && !AFF; // geometry streamed per point
-  static_assert(!AFF || (!LIT && !REF && !BLK), "affine geometry variants");
-  static_assert(!EO || (!LIT && !REF && !BLK), "even-odd variants");
+  static_assert(!AFF || (!LIT && !REF && !WL), "affine geometry variants");
+  static_assert(!EO || (!LIT && !REF && !WL), "even-odd variants");
   constexpr bool UQ = MASSTERM || LIT;
   // even-odd table set: term offsets, the weights, half lengths
   constexpr int EPF = eo_offset(N, Q, EO_PHI_F), EGF = eo_offset(N, Q, EO_DPHI_F);
@@ -410,18 +460,18 @@ apply_body(const T* __restrict__ tab, const T* __restrict__ x,
   constexpr int Y_RA = 0, Y_RB = 1, Y_RC = 2;  // B3 -> B2
   constexpr int X_S0 = 0, X_SB = 1;            // B2 -> B1
 
-  __shared__ Piece<T> Ls_static[BLK ? 1 : LDSW];
+  __shared__ Piece<T> Ls_static[WL ? 1 : LDSW];
   extern __shared__ __align__(16) unsigned char femtc_lds_dyn[];
   Piece<T>* Ls_dyn = reinterpret_cast<Piece<T>*>(femtc_lds_dyn);
   // this wave's (or, unblocked, the workgroup's) work area
-  const int wave = BLK ? (int)(threadIdx.x >> 6) : 0;
-  Piece<T>* Ls = BLK ? Ls_dyn + wave * LDSW : Ls_static;
+  const int wave = WL ? (int)(threadIdx.x >> 6) : 0;
+  Piece<T>* Ls = WL ? Ls_dyn + wave * LDSW : Ls_static;
   T* L = reinterpret_cast<T*>(Ls);
 
   const T* __restrict__ ph = tab;
   const T* __restrict__ gr = tab + N * Q;
 
-  const int tid = BLK ? (int)(threadIdx.x & 63) : (int)threadIdx.x;
+  const int tid = WL ? (int)(threadIdx.x & 63) : (int)threadIdx.x;
   const int lc = tid / NL;
   const int ln = tid - lc * NL;
   const bool lane_ok = lc < CPB;
@@ -486,6 +536,13 @@ apply_body(const T* __restrict__ tab, const T* __restrict__ x,
         cell0 = (ci * ba.ncy + cj) * ba.ncz + ck0;
       }
     }
+    else if constexpr (ZR)
+    {
+      // this wave's CPB cells of the segment's zncell cells from zcell0
+      cell0 = zcell0 + wave * CPB;
+      const long long left = zcell0 + zncell - cell0;
+      ncells_blk = left < CPB ? (left > 0 ? left : 0) : (long long)CPB;
+    }
     else
     {
       cell0 = blk * CPB;
@@ -498,17 +555,20 @@ apply_body(const T* __restrict__ tab, const T* __restrict__ x,
 
     // ---------------- global loads issued up front ------------------------
     T u[N];
-    int32_t idx[GATHER ? N : 1];
+    int32_t idx[GATHER && !CLOSED ? N : 1];
+    // structured scatters: the dofmap is the structured numbering (verified
+    // by the caller), so the gather (and STR's scatter) indices are
+    // sbase + b * sstep, computed, not loaded
+    long long sbase = 0, sstep = 0;
     if (live && role1)
     {
-      // assembled: the dofmap is the structured numbering (verified by the
-      // caller), so the gather indices are computed, not loaded
-      long long sbase = 0, sstep = 0;
-      if constexpr (ASM)
+      if constexpr (CLOSED)
       {
-        const long long ck = cell % ba.ncz;
-        const long long t = cell / ba.ncz;
-        const long long cj = t % ba.ncy, ci = t / ba.ncy;
+        // 32-bit division: the launchers keep num_cells below 2^31
+        const unsigned cu = (unsigned)cell, t = cu / (unsigned)ba.ncz;
+        const long long ck = cu - t * (unsigned)ba.ncz;
+        const unsigned ti = t / (unsigned)ba.ncy;
+        const long long cj = t - ti * (unsigned)ba.ncy, ci = ti;
         constexpr int P1 = N - 1;
         sstep = (long long)ba.ncz * P1 + 1;
         sbase = ((ci * P1 + r1a) * ((long long)ba.ncy * P1 + 1) + cj * P1) *
@@ -518,7 +578,7 @@ apply_body(const T* __restrict__ tab, const T* __restrict__ x,
 #pragma unroll
       for (int b = 0; b < N; ++b)
       {
-        if constexpr (ASM)
+        if constexpr (CLOSED)
           u[b] = x[sbase + b * sstep];
         else if constexpr (GATHER)
         {
@@ -621,7 +681,7 @@ apply_body(const T* __restrict__ tab, const T* __restrict__ x,
           Xc[X_G * SX + q * XQ + r1c * XC + r1a] = ag[q];
       }
     }
-    stage_sync<BLK>();
+    stage_sync<WL>();
 
     // ---------------- F2: contract axis c ---------------------------------
     T bpp[Q], bpg[Q], bgp[Q];
@@ -673,7 +733,7 @@ apply_body(const T* __restrict__ tab, const T* __restrict__ x,
         }
       }
     }
-    stage_sync<BLK>(); // X is read; Y overwrites the same LDS
+    stage_sync<WL>(); // X is read; Y overwrites the same LDS
     if (live && role2)
     {
       const int o = r2a * YA + r2q * YQ;
@@ -688,7 +748,7 @@ apply_body(const T* __restrict__ tab, const T* __restrict__ x,
         }
       }
     }
-    stage_sync<BLK>();
+    stage_sync<WL>();
 
     // ---------------- F3 + quadrature-point work + B3 ---------------------
     T ipp[N], ipg[N], igp[N];
@@ -705,7 +765,7 @@ apply_body(const T* __restrict__ tab, const T* __restrict__ x,
         }
       }
     }
-    stage_sync<BLK>(); // X and Y are idle from here until B3 writes Y
+    stage_sync<WL>(); // X and Y are idle from here until B3 writes Y
     // first geometry half into LDS (its registers are freed before F3)
     if constexpr (GEO)
     {
@@ -716,7 +776,7 @@ apply_body(const T* __restrict__ tab, const T* __restrict__ x,
         if (p < GP0)
           Ls[p] = gp0[k];
       }
-      stage_sync<BLK>();
+      stage_sync<WL>();
     }
 
     // F3: contract axis a
@@ -832,7 +892,7 @@ apply_body(const T* __restrict__ tab, const T* __restrict__ x,
     }
     if constexpr (GEO)
     {
-      stage_sync<BLK>();
+      stage_sync<WL>();
 #pragma unroll
       for (int k = 0; k < GPT1; ++k)
       {
@@ -840,7 +900,7 @@ apply_body(const T* __restrict__ tab, const T* __restrict__ x,
         if (p < GP1)
           Ls[p] = gp1[k];
       }
-      stage_sync<BLK>();
+      stage_sync<WL>();
     }
     T ra_[N], rb_[N], rc_[N]; // B3 outputs, stored after the geometry is read
     if (live)
@@ -910,9 +970,9 @@ apply_body(const T* __restrict__ tab, const T* __restrict__ x,
       }
     }
     if constexpr (GEO)
-      stage_sync<BLK>(); // the geometry in LDS is consumed; Y may be rewritten
+      stage_sync<WL>(); // the geometry in LDS is consumed; Y may be rewritten
     // energy partial of this wave (fixed order: lane tree, one slot per wave)
-    if constexpr (!BLK && !LIT)
+    if constexpr (!WL && !LIT)
     {
       if (epart)
       {
@@ -937,7 +997,7 @@ apply_body(const T* __restrict__ tab, const T* __restrict__ x,
           Yc[Y_RC * SY + o] = rc_[a];
       }
     }
-    stage_sync<BLK>();
+    stage_sync<WL>();
 
     // ---------------- B2: contract axis c (transposed) --------------------
     T ra[Q], rb[Q], rc[Q];
@@ -955,7 +1015,7 @@ apply_body(const T* __restrict__ tab, const T* __restrict__ x,
           rc[q] = Yc[Y_RC * SY + o + q];
       }
     }
-    stage_sync<BLK>(); // Y is read; X overwrites the same LDS
+    stage_sync<WL>(); // Y is read; X overwrites the same LDS
     if (live && role2 && EO)
     {
       T s0[N], sb[N], ev[NP], od[NH];
@@ -1013,7 +1073,7 @@ apply_body(const T* __restrict__ tab, const T* __restrict__ x,
           Xc[X_SB * SX + xo] = sb;
       }
     }
-    stage_sync<BLK>();
+    stage_sync<WL>();
 
     // ---------------- B1: contract axis b (transposed) + store/scatter ----
     if (live && role1)
@@ -1030,9 +1090,10 @@ apply_body(const T* __restrict__ tab, const T* __restrict__ x,
       long long wpos = 0, wrow = 0;
       if constexpr (ASM)
       {
-        const long long ck = cell % ba.ncz;
-        const long long t = cell / ba.ncz;
-        const long long cj = t % ba.ncy, ci = t / ba.ncy;
+        const unsigned cu = (unsigned)cell, t = cu / (unsigned)ba.ncz;
+        const long long ck = cu - t * (unsigned)ba.ncz;
+        const unsigned ti = t / (unsigned)ba.ncy;
+        const long long cj = t - ti * (unsigned)ba.ncy, ci = ti;
         wrow = (long long)ba.ncz * N;
         wpos = ((ci * N + r1a) * ba.ncy + cj) * N * wrow + ck * N + r1c;
       }
@@ -1087,6 +1148,14 @@ apply_body(const T* __restrict__ tab, const T* __restrict__ x,
         }
         else if constexpr (ASM)
           y[wpos + b * wrow] = w;
+        else if constexpr (STR)
+          atomic_add(y + sbase + b * sstep, w);
+        else if constexpr (ZR)
+        {
+          constexpr int ZB = zrun_cells<T, N, Q, OP>() * P + 1;
+          lds_add(zbox + (r1a * N + b) * ZB + (int)(cell - zcell0) * P + r1c,
+                  w);
+        }
         else if constexpr (GATHER)
           atomic_add(y + idx[b], w);
         else
@@ -1153,7 +1222,9 @@ constexpr int min_waves_per_simd()
 template <typename T, int N, int Q, int OP, bool REF, bool GATHER, int BW,
           bool EO = false>
 __global__ void
-__launch_bounds__((BW > 0 ? 64 * BW : group_threads<T, Q, BW>()),
+__launch_bounds__((BW > 0                ? 64 * BW
+                   : BW == SCATTER_ZRUN ? 64 * zrun_waves<T, N, Q, OP>()
+                                        : group_threads<T, Q, BW>()),
                   (min_waves_per_simd<T, Q, OP, BW>()))
     apply_cells(const T* __restrict__ tab, const T* __restrict__ x,
                 T* __restrict__ y, const T* __restrict__ G,
@@ -1174,12 +1245,65 @@ __launch_bounds__((BW > 0 ? 64 * BW : group_threads<T, Q, BW>()),
     if (blk >= ba.nblk)
       return; // uniform over the workgroup
   }
-  if constexpr (BW == SCATTER_ASSEMBLED)
+  if constexpr (BW == SCATTER_ZRUN)
+  {
+    // One z-segment of zrun_cells cells (fewer at the end of a z-line) per
+    // workgroup, segments numbered (ci*ncy + cj)*nzs + zs and dealt to the
+    // XCDs in contiguous ranges as below.  Each wave takes CPB cells of the
+    // segment in its own LDS work area; their values are summed into the
+    // (a, b, z) box of the segment's dofs in LDS, and the box is then added
+    // to y with atomics, consecutive lanes on consecutive global dofs
+    // along z.
+    constexpr int P = N - 1;
+    constexpr int S = zrun_cells<T, N, Q, OP>();
+    constexpr int ZB = S * P + 1;
+    constexpr int BOXN = N * N * ZB;
+    constexpr int BT = 64 * zrun_waves<T, N, Q, OP>();
+    const long long per = (ba.nblk + 7) / 8;
+    const long long seg = (long long)(blockIdx.x % 8) * per + blockIdx.x / 8;
+    if (seg >= ba.nblk)
+      return; // uniform over the workgroup
+    extern __shared__ __align__(16) unsigned char femtc_lds_dyn[];
+    T* zbox = reinterpret_cast<T*>(femtc_lds_dyn +
+                                   zrun_waves<T, N, Q, OP>() *
+                                       blocked_work_bytes<T, N, Q, OP>());
+    const unsigned nzs = ((unsigned)ba.ncz + S - 1) / S;
+    const unsigned line = (unsigned)seg / nzs; // ci*ncy + cj
+    const int ck0 = (int)((unsigned)seg - line * nzs) * S;
+    const int ncell = ba.ncz - ck0 < S ? ba.ncz - ck0 : S;
+    const long long zcell0 = (long long)line * ba.ncz + ck0;
+    for (int i = threadIdx.x; i < BOXN; i += BT)
+      zbox[i] = T(0);
+    __syncthreads();
+    apply_body<T, N, Q, OP, REF, GATHER, BW, EO>(
+        tab, x, y, G, detJw, dofmap, num_cells, ba, 0, nullptr, zbox, zcell0,
+        ncell);
+    __syncthreads(); // every wave's box sums are in
+    const unsigned ci = line / (unsigned)ba.ncy;
+    const unsigned cj = line - ci * (unsigned)ba.ncy;
+    const long long NYg = (long long)ba.ncy * P + 1;
+    const long long NZg = (long long)ba.ncz * P + 1;
+    const int zlen = ncell * P + 1;
+    const long long gz0 = (long long)ck0 * P;
+    for (int i = threadIdx.x; i < BOXN; i += BT)
+    {
+      const int ab = i / ZB, z = i - ab * ZB;
+      if (z < zlen)
+      {
+        const int a = ab / N, b = ab - a * N;
+        atomic_add(y + ((ci * P + a) * NYg + cj * P + b) * NZg + gz0 + z,
+                   zbox[i]);
+      }
+    }
+    return;
+  }
+  if constexpr (BW == SCATTER_ASSEMBLED || BW == SCATTER_STRUCT)
   {
     // Workgroups are dealt to the 8 XCDs round-robin; give each XCD a
     // contiguous range of cell blocks so that the partial lines its stores
-    // write meet in its own L2 (a layout choice only: any mapping of
-    // workgroups to blocks is correct).
+    // write, and the x lines neighbouring cells share, meet in its own L2
+    // (a layout choice only: any mapping of workgroups to blocks is
+    // correct).
     const long long per = (ba.nblk + 7) / 8;
     blk = (long long)(blockIdx.x % 8) * per + blockIdx.x / 8;
     if (blk >= ba.nblk)
@@ -1372,7 +1496,8 @@ int launch_assembled(const ApplyReq& r)
 {
   constexpr int CPB = cells_per_group<T, Q, SCATTER_ASSEMBLED>();
   if (r.ncx < 1 || r.ncy < 1 || r.ncz < 1 || !r.lattice ||
-      (long long)r.ncx * r.ncy * r.ncz != r.num_cells)
+      (long long)r.ncx * r.ncy * r.ncz != r.num_cells ||
+      r.num_cells > 0x7fffffffLL)
     return FEMTC_ERR_INVALID_ARGUMENT;
   const long long nblk = (r.num_cells + CPB - 1) / CPB;
   if (nblk > 0x7ffffff0LL)
@@ -1401,6 +1526,67 @@ int launch_assembled(const ApplyReq& r)
         r.overwrite ? 0 : 1, r.skip, r.stream);
   }
   return hip_status(hipGetLastError());
+}
+
+// Structured direct scatter: the atomic scatter with closed-form indices,
+// workgroups taken XCD-chunked as in the assembled cell pass.
+template <typename T, int N, int Q, int OP>
+int launch_struct(const ApplyReq& r)
+{
+  constexpr int CPB = cells_per_block<Q>();
+  if (r.ncx < 1 || r.ncy < 1 || r.ncz < 1 ||
+      (long long)r.ncx * r.ncy * r.ncz != r.num_cells ||
+      r.num_cells > 0x7fffffffLL)
+    return FEMTC_ERR_INVALID_ARGUMENT;
+  const long long nblk = (r.num_cells + CPB - 1) / CPB;
+  BlockArgs ba{};
+  ba.ncx = r.ncx;
+  ba.ncy = r.ncy;
+  ba.ncz = r.ncz;
+  ba.nblk = (int)nblk;
+  const long long grid = 8 * ((nblk + 7) / 8);
+  hipLaunchKernelGGL((apply_cells<T, N, Q, OP, false, true, SCATTER_STRUCT>),
+                     dim3((unsigned)grid), dim3(block_threads<Q>()), 0,
+                     r.stream, (const T*)r.tab, (const T*)r.x, (T*)r.y,
+                     (const T*)r.G, (const T*)r.detJw, r.dofmap,
+                     (long long)r.num_cells, r.skip, ba, r.energy);
+  return hip_status(hipGetLastError());
+}
+
+// Structured z-run scatter: one workgroup per z-segment of cells.
+template <typename T, int N, int Q, int OP>
+int launch_zrun(const ApplyReq& r)
+{
+  if constexpr (Q * Q > 64)
+    return FEMTC_ERR_INVALID_ARGUMENT;
+  else
+  {
+    constexpr int S = zrun_cells<T, N, Q, OP>();
+    if (r.ncx < 1 || r.ncy < 1 || r.ncz < 1 ||
+        (long long)r.ncx * r.ncy * r.ncz != r.num_cells ||
+        r.num_cells > 0x7fffffffLL)
+      return FEMTC_ERR_INVALID_ARGUMENT;
+    const long long nseg =
+        (long long)r.ncx * r.ncy * ((r.ncz + S - 1) / S);
+    if (nseg > 0x7ffffff0LL)
+      return FEMTC_ERR_INVALID_ARGUMENT;
+    BlockArgs ba{};
+    ba.ncx = r.ncx;
+    ba.ncy = r.ncy;
+    ba.ncz = r.ncz;
+    ba.nblk = (int)nseg;
+    const long long grid = 8 * ((nseg + 7) / 8);
+    constexpr int W = zrun_waves<T, N, Q, OP>();
+    constexpr size_t lds = W * blocked_work_bytes<T, N, Q, OP>() +
+                           sizeof(T) * N * N * (S * (N - 1) + 1);
+    static_assert(lds <= 64 * 1024, "z-run workgroup LDS");
+    hipLaunchKernelGGL((apply_cells<T, N, Q, OP, false, true, SCATTER_ZRUN>),
+                       dim3((unsigned)grid), dim3(64 * W), lds, r.stream,
+                       (const T*)r.tab, (const T*)r.x, (T*)r.y, (const T*)r.G,
+                       (const T*)r.detJw, r.dofmap, (long long)r.num_cells,
+                       r.skip, ba, nullptr);
+    return hip_status(hipGetLastError());
+  }
 }
 
 // Even-odd instances (Q == N only): direct or assembled scatter.
@@ -1445,6 +1631,30 @@ int dispatch_op(const ApplyReq& r)
     case OP_MASS_AFF: return launch_assembled<T, N, Q, OP_MASS_AFF>(r);
     case OP_STIFF_AFF: return launch_assembled<T, N, Q, OP_STIFF_AFF>(r);
     case OP_KPM_AFF: return launch_assembled<T, N, Q, OP_KPM_AFF>(r);
+    }
+    return FEMTC_ERR_INVALID_ARGUMENT;
+  }
+  if (r.zrun)
+  {
+    if (!g || r.reference_mode || r.structured_direct)
+      return FEMTC_ERR_INVALID_ARGUMENT;
+    switch (r.op)
+    {
+    case OP_MASS: return launch_zrun<T, N, Q, OP_MASS>(r);
+    case OP_STIFF: return launch_zrun<T, N, Q, OP_STIFF>(r);
+    case OP_KPM: return launch_zrun<T, N, Q, OP_KPM>(r);
+    }
+    return FEMTC_ERR_INVALID_ARGUMENT;
+  }
+  if (r.structured_direct)
+  {
+    if (!g || r.reference_mode)
+      return FEMTC_ERR_INVALID_ARGUMENT;
+    switch (r.op)
+    {
+    case OP_MASS: return launch_struct<T, N, Q, OP_MASS>(r);
+    case OP_STIFF: return launch_struct<T, N, Q, OP_STIFF>(r);
+    case OP_KPM: return launch_struct<T, N, Q, OP_KPM>(r);
     }
     return FEMTC_ERR_INVALID_ARGUMENT;
   }

@@ -70,6 +70,14 @@ struct ApplyReq
   // overwrite, else y += sum), assembled = 2 stops after the cell pass.
   int assembled = 0;
   void* lattice = nullptr;
+  // Structured direct scatter (gather only, ncx..ncz as above, the dofmap
+  // verified to be the structured one): closed-form gather indices and the
+  // atomic scatter at the same indices; the dofmap is not read.
+  int structured_direct = 0;
+  // Structured z-run scatter (gather only, as structured_direct): each
+  // workgroup sums a segment of cells along z in LDS and adds the segment's
+  // dof box to y with atomics in runs of consecutive global dofs.
+  int zrun = 0;
   // Even-odd table set (even_odd.hpp) of symmetric tables, used instead of
   // tab when set: Q == N, gather, correct mode, direct or assembled scatter.
   const void* eo_tab = nullptr;
@@ -77,6 +85,12 @@ struct ApplyReq
 
 // BW template value of the assembled scatter's cell-pass instances.
 constexpr int SCATTER_ASSEMBLED = -1;
+// BW template value of the structured direct scatter: the dofmap verified to
+// be the structured numbering, the gather and the atomic scatter use the
+// global indices in closed form instead of reading the dofmap.
+constexpr int SCATTER_STRUCT = -2;
+// BW template value of the structured z-run scatter (ApplyReq::zrun).
+constexpr int SCATTER_ZRUN = -3;
 
 // Cells per workgroup of the assembled scatter's cell pass (esize: bytes
 // per value).  One wave of whole cells (Q^2 threads each) leaves lanes idle

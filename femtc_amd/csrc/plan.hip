@@ -294,7 +294,8 @@ struct femtc_plan
   // scatter is first selected or timed)
   void* lattice[2] = {nullptr, nullptr};
   // chosen block shape per (nq, dtype, op); x == 0 means direct scatter,
-  // x < 0 the assembled scatter
+  // x == -1 the assembled scatter, x == -2 the structured direct scatter,
+  // x == -3 the structured z-run scatter
   std::vector<std::pair<int, Shape>> shapes;
   // tables registered by femtc_plan_set_tables, per (dtype, nq): a copy,
   // and its even-odd table set when the tables are symmetric (else null)
@@ -369,7 +370,7 @@ int plan_request(const femtc_plan* pl, int op, int dtype, int nq,
   r.gather = 1;
   r.tab = tables;
   // the even-odd instances cover the direct and assembled scatters
-  r.eo_tab = s.x <= 0 && nq == r.N ? eo : nullptr;
+  r.eo_tab = (s.x == 0 || s.x == -1) && nq == r.N ? eo : nullptr;
   r.x = x;
   r.y = y;
   r.G = G;
@@ -377,7 +378,15 @@ int plan_request(const femtc_plan* pl, int op, int dtype, int nq,
   r.dofmap = pl->dofmap;
   r.num_cells = pl->num_cells;
   r.stream = st;
-  if (s.x < 0)
+  if (s.x == -2 || s.x == -3)
+  {
+    r.structured_direct = s.x == -2;
+    r.zrun = s.x == -3;
+    r.ncx = pl->nx;
+    r.ncy = pl->ny;
+    r.ncz = pl->nz;
+  }
+  else if (s.x < 0)
   {
     r.assembled = 1;
     r.lattice = pl->lattice[dtype == FEMTC_F64 ? 0 : 1];
@@ -452,10 +461,14 @@ int launch_plan(const femtc_plan* pl, int op, int dtype, int nq,
                 const void* G, const void* detJw, int accumulate, Shape s,
                 hipStream_t st)
 {
-  if (s.x < 0 && !pl->lattice[dtype == FEMTC_F64 ? 0 : 1])
+  if (s.x == -1 && !pl->lattice[dtype == FEMTC_F64 ? 0 : 1])
     return fail(FEMTC_ERR_INVALID_ARGUMENT,
                 "assembled scatter selected without its buffer");
-  if (!accumulate && s.x >= 0) // the assembly pass overwrites by itself
+  if ((s.x == -2 || s.x == -3) && !pl->structured)
+    return fail(FEMTC_ERR_INVALID_ARGUMENT,
+                "structured direct scatter: the dofmap is not the structured "
+                "numbering of the plan's grid");
+  if (!accumulate && s.x != -1) // the assembly pass overwrites by itself
   {
     int r0 = fill_zero(y, pl->ndofs, dtype, st);
     if (r0 != FEMTC_OK)
@@ -585,6 +598,27 @@ int femtc_plan_set_block(femtc_plan* plan, int nq, int dtype, int op,
     plan->set(nq, dtype, op, Shape{-1, -1, -1});
     return FEMTC_OK;
   }
+  if (cbx == -2 && cby == -2 && cbz == -2)
+  {
+    if (!plan->structured)
+      return fail(FEMTC_ERR_INVALID_ARGUMENT,
+                  "structured direct scatter: the dofmap is not the "
+                  "structured numbering of the plan's grid");
+    plan->set(nq, dtype, op, Shape{-2, -2, -2});
+    return FEMTC_OK;
+  }
+  if (cbx == -3 && cby == -3 && cbz == -3)
+  {
+    if (!plan->structured)
+      return fail(FEMTC_ERR_INVALID_ARGUMENT,
+                  "structured z-run scatter: the dofmap is not the "
+                  "structured numbering of the plan's grid");
+    if (nq > 8)
+      return fail(FEMTC_ERR_INVALID_ARGUMENT,
+                  "structured z-run scatter: nq <= 8 only");
+    plan->set(nq, dtype, op, Shape{-3, -3, -3});
+    return FEMTC_OK;
+  }
   const int cpw = cells_per_wave(nq);
   const int cells = cbx * cby * cbz;
   if (cpw == 0 || cbx < 1 || cby < 1 || cbz < 1 || cbz % cpw != 0 ||
@@ -689,6 +723,10 @@ int femtc_plan_autotune(femtc_plan* plan, int op, int dtype, int nq,
   hipStream_t s = (hipStream_t)stream;
   std::vector<Shape> cand = candidates(nq);
   cand.push_back(Shape{0, 0, 0}); // direct scatter
+  if (plan->structured)
+    cand.push_back(Shape{-2, -2, -2}); // structured direct scatter
+  if (plan->structured && nq <= 8)
+    cand.push_back(Shape{-3, -3, -3}); // structured z-run scatter
   if (plan->structured && ensure_lattice(plan, dtype) == FEMTC_OK)
     cand.push_back(Shape{-1, -1, -1}); // assembled scatter
   hipEvent_t e0, e1;

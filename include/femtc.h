@@ -191,6 +191,16 @@ int femtc_global_apply_add(int op, int dtype, int degree, int nq,
 /* y[0..n) = 0 with a kernel (usable under graph capture). */
 int femtc_fill_zero(void* y, int64_t n, int dtype, void* stream);
 
+/* Streaming probe (diagnostic; no counterpart in the reference): the
+ * practical HBM bandwidth of a read-dominated stream like the operator
+ * apply.  dst[i] = sum_r src[r*n + i] over nread (1..8) arrays of n (even)
+ * doubles when nwrite = 1; with nwrite = 0 the sums land in sink (one per
+ * workgroup, nsink >= 1 values).  16-byte non-temporal loads, grid-stride
+ * over wg_per_cu (1..64; 0: 2) workgroups of 256 lanes per compute unit. */
+int femtc_stream_probe(int nread, int nwrite, const double* src, double* dst,
+                       int64_t n, double* sink, int nsink, int wg_per_cu,
+                       void* stream);
+
 /* ---------------------------------------------------------------------- */
 /* Affine cells, geometry computed in the kernel (no counterpart in the    */
 /* reference, which streams G per point): per cell 7 values cellgeo =      */
@@ -314,10 +324,15 @@ int femtc_plan_create(femtc_plan** plan, int nx, int ny, int nz, int degree,
                       const int32_t* dofmap, int64_t ndofs, void* stream);
 int femtc_plan_destroy(femtc_plan* plan);
 /* Block shape (cells per axis) for one (nq, dtype, op); 0,0,0 selects the
- * direct atomic scatter and -1,-1,-1 the assembled scatter (refused with
- * FEMTC_ERR_INVALID_ARGUMENT when the dofmap is not the structured one).
- * cbz must be a multiple of 64/nq^2 and the block at most 8 waves of
- * 64/nq^2 cells; nq <= 8. */
+ * direct atomic scatter, -1,-1,-1 the assembled scatter, -2,-2,-2 the
+ * structured direct scatter (the atomic scatter with the gather and scatter
+ * indices in closed form: the dofmap is not read) and -3,-3,-3 the
+ * structured z-run scatter (closed-form indices; each workgroup sums a run
+ * of cells along z in LDS and adds their dofs to y with atomics on runs of
+ * consecutive dofs; nq <= 8); the last three are refused with
+ * FEMTC_ERR_INVALID_ARGUMENT when the dofmap is not the structured one.
+ * For blocks, cbz must be a multiple of 64/nq^2 and the block at most 8
+ * waves of 64/nq^2 cells; nq <= 8. */
 int femtc_plan_set_block(femtc_plan* plan, int nq, int dtype, int op, int cbx,
                          int cby, int cbz);
 int femtc_plan_get_block(const femtc_plan* plan, int nq, int dtype, int op,
@@ -342,7 +357,8 @@ int femtc_plan_apply(const femtc_plan* plan, int op, int dtype, int nq,
 int femtc_plan_set_tables(femtc_plan* plan, int dtype, int nq,
                           const void* tables, void* stream, int* even_odd);
 /* Time every candidate block shape (and the direct and, for the structured
- * dofmap, the assembled scatter) on these buffers, reps overwriting applies
+ * dofmap, the structured direct, z-run and assembled scatters) on these
+ * buffers, reps overwriting applies
  * each (y = A x, clearing included), and keep the fastest for
  * (nq, dtype, op).  y is left cleared.  best_us (may be NULL) receives its
  * time per apply.  tables may be NULL as for femtc_plan_apply (the

@@ -143,6 +143,45 @@ def test_plan_assembled_vs_oracle(op, nx, ny, nz, P, Q):
     assert rel(host(dy), ref + y0) <= 1e-12
 
 
+STRUCTURED_CASES = (
+    [(m, (-2, -2, -2)) for m in ASSEMBLED_MESHES] +
+    [(m, (-3, -3, -3)) for m in ASSEMBLED_MESHES + [(2, 2, 9, 4, 5),
+                                                    (2, 3, 7, 6, 7),
+                                                    (2, 2, 17, 1, 2),
+                                                    (1, 2, 12, 2, 3)]
+     if m[4] <= 8])
+
+
+@pytest.mark.parametrize("mesh,shape", STRUCTURED_CASES)
+@pytest.mark.parametrize("op", [F.OP_MASS, F.OP_STIFFNESS, F.OP_KPM])
+def test_plan_structured_scatters_vs_oracle(op, mesh, shape):
+    # -2: the atomic scatter with closed-form gather and scatter indices;
+    # -3: closed-form gather, cells summed along z in LDS, atomics on runs
+    # of consecutive dofs (segments shorter at the end of a z-line)
+    nx, ny, nz, P, Q = mesh
+    C, G, Dw, dm, ng = _box(nx, ny, nz, P, Q, 85 + P)
+    tab = O.basis_tables(P, Q)
+    x = O.uniform(86, ng)
+    ref = O.global_operator(op, P + 1, Q, tab, x, dm, G, Dw)
+    plan = F.Plan(nx, ny, nz, P, dev(dm, torch.int32), ng)
+    dtab, dx, dG, dD = dev(tab), dev(x), dev(G), dev(Dw)
+    plan.set_block(Q, F.F64, op, shape)
+    assert plan.get_block(Q, F.F64, op) == shape
+    dy = torch.full((ng,), 5.0, device="cuda", dtype=torch.float64)
+    plan.apply(op, dtab, dx, dy, Q, G=dG, detJw=dD)
+    assert rel(host(dy), ref) <= 1e-12
+    y0 = O.uniform(87, ng)
+    dy = dev(y0)
+    plan.apply(op, dtab, dx, dy, Q, G=dG, detJw=dD, accumulate=True)
+    assert rel(host(dy), ref + y0) <= 1e-12
+    f = torch.float32
+    plan.set_block(Q, F.F32, op, shape)
+    dy = torch.empty(ng, device="cuda", dtype=f)
+    plan.apply(op, dev(tab, f), dev(x, f), dy, Q, G=dev(G, f),
+               detJw=dev(Dw, f))
+    assert rel(host(dy), ref) <= 1e-5
+
+
 @pytest.mark.parametrize("P,Q", [(2, 3), (4, 5), (6, 7), (8, 9)])
 def test_plan_assembled_f32(P, Q):
     nx, ny, nz = 3, 4, 5
@@ -186,10 +225,14 @@ def test_plan_assembled_only_for_the_structured_numbering(kind):
     plan = F.Plan(nx, ny, nz, P, dev(dm, torch.int32), ng)
     with pytest.raises(F.FemtcError, match="structured"):
         plan.set_block(Q, F.F64, F.OP_KPM, (-1, -1, -1))
+    with pytest.raises(F.FemtcError, match="structured"):
+        plan.set_block(Q, F.F64, F.OP_KPM, (-2, -2, -2))
+    with pytest.raises(F.FemtcError, match="structured"):
+        plan.set_block(Q, F.F64, F.OP_KPM, (-3, -3, -3))
     dy = torch.empty(ng, device="cuda", dtype=torch.float64)
     shape, _ = plan.autotune(F.OP_KPM, dev(tab), dev(x), dy, Q, G=dev(G),
                              detJw=dev(Dw), reps=1)
-    assert shape != (-1, -1, -1)
+    assert shape not in ((-1, -1, -1), (-2, -2, -2), (-3, -3, -3))
     plan.apply(F.OP_KPM, dev(tab), dev(x), dy, Q, G=dev(G), detJw=dev(Dw))
     assert rel(host(dy), ref) <= 1e-12
 
@@ -303,3 +346,13 @@ def test_plan_autotune_with_registered_tables():
     assert us > 0 and float(dy.abs().max()) == 0.0
     plan.apply(F.OP_STIFFNESS, None, dx, dy, Q, G=dG)
     assert rel(host(dy), ref) <= 1e-12, shape
+
+
+def test_plan_zrun_scatter_limits():
+    # the z-run scatter holds one-wave cell groups: nq <= 8 only
+    nx, ny, nz, P = 2, 2, 2, 8
+    C, G, Dw, dm, ng = _box(nx, ny, nz, P, 9, 31)
+    plan = F.Plan(nx, ny, nz, P, dev(dm, torch.int32), ng)
+    with pytest.raises(F.FemtcError, match="nq <= 8"):
+        plan.set_block(9, F.F64, F.OP_MASS, (-3, -3, -3))
+    plan.set_block(9, F.F64, F.OP_MASS, (-2, -2, -2))
