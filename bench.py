@@ -264,14 +264,16 @@ def bench_solve(torch, F, n=48, P=6):
     """(K+M)u = f to 1e-10: the plain FP64 PCG (the reference's data: per-
     point geometry, dofmap, atomic scatter) against FP32 inner PCG + FP64
     refinement, each also with per-cell geometry (affine cells) and with the
-    structured grid given (assembled scatter when it times faster)."""
+    structured grid given (the assembled or z-run scatter when it times
+    faster)."""
     pr = structured(torch, F, n, P, want_detJw=True)
     f = torch.from_numpy(_uniform(7, pr["ng"], -1, 1)).cuda()
     out = {"workload": f"(K+M)u=f, {n}^3 mesh, P={P}, Q={P + 1}, f~U[-1,1], "
                        "rtol 1e-10 (FP64 residual)"}
     sols = {}
     grid = (n, n, n)
-    names = {F.SCATTER_DIRECT: "atomic", F.SCATTER_ASSEMBLED: "assembled"}
+    names = {F.SCATTER_DIRECT: "atomic", F.SCATTER_ASSEMBLED: "assembled",
+             F.SCATTER_ZRUN: "z-run"}
     inner_names = {F.F64: "fp64", F.F32: "fp32", F.BF16: "bf16_mfma_split",
                    F.BF16_ROUNDED: "bf16_mfma_rounded"}
     for name, inner, J, g, irtol in (

@@ -6,7 +6,7 @@ library; the first call does, and fails loudly if it is not built.
 """
 from ._lib import (BF16, BF16_ROUNDED, F32, F64, MODE_CORRECT, MODE_REFERENCE, OP_KPM,
                    OP_MASS, OP_STIFFNESS, SCATTER_ASSEMBLED, SCATTER_AUTO,
-                   SCATTER_DIRECT, FemtcError, lib)
+                   SCATTER_DIRECT, SCATTER_ZRUN, FemtcError, lib)
 from .linalg import Order, string2order
 from .mesh import (basis_tables, basis_tables_weights, dofmap_structured,
                    even_odd_tables, gauss_rule, geometry_affine,
@@ -25,6 +25,7 @@ __all__ = [
     "global_diagonal", "latency_stats",
     "BF16", "BF16_ROUNDED", "F32", "F64", "MODE_CORRECT", "MODE_REFERENCE", "OP_KPM", "OP_MASS",
     "OP_STIFFNESS", "SCATTER_ASSEMBLED", "SCATTER_AUTO", "SCATTER_DIRECT",
+    "SCATTER_ZRUN",
     "FemtcError", "lib", "Order", "string2order", "fill_zero",
     "basis_tables", "dofmap_structured", "gauss_rule", "geometry_affine",
     "gll_points", "global_apply", "kpm_apply", "mass_apply", "mass_operator",

@@ -138,7 +138,7 @@ class Problem(C.Structure):
                 ("grid", _i32 * 3), ("scatter", _i32), ("even_odd", _i32)]
 
 
-SCATTER_AUTO, SCATTER_DIRECT, SCATTER_ASSEMBLED = 0, 1, 2
+SCATTER_AUTO, SCATTER_DIRECT, SCATTER_ASSEMBLED, SCATTER_ZRUN = 0, 1, 2, 3
 
 
 class BatchItem(C.Structure):

@@ -421,7 +421,7 @@ apply_body(const T* __restrict__ tab, const T* __restrict__ x,
   constexpr bool GRAD = (OPB != OP_MASS);
   constexpr bool FULL = (OPB == OP_STIFF || OPB == OP_KPM); // 3 gradients
   constexpr bool GEO = GRAD && !AFF; // geometry streamed per point
-  static_assert(!AFF || (!LIT && !REF && !WL), "affine geometry variants");
+  static_assert(!AFF || (!LIT && !REF && !BLK), "affine geometry variants");
   static_assert(!EO || (!LIT && !REF && !WL), "even-odd variants");
   constexpr bool UQ = MASSTERM || LIT;
   // even-odd table set: term offsets, the weights, half lengths
@@ -971,8 +971,9 @@ apply_body(const T* __restrict__ tab, const T* __restrict__ x,
     }
     if constexpr (GEO)
       stage_sync<WL>(); // the geometry in LDS is consumed; Y may be rewritten
-    // energy partial of this wave (fixed order: lane tree, one slot per wave)
-    if constexpr (!WL && !LIT)
+    // energy partial of this wave (fixed order: lane tree, one slot per
+    // wave of workgroup blk)
+    if constexpr (!BLK && !LIT)
     {
       if (epart)
       {
@@ -980,7 +981,7 @@ apply_body(const T* __restrict__ tab, const T* __restrict__ x,
         for (int o = 32; o > 0; o >>= 1)
           eacc += __shfl_xor(eacc, o, 64);
         if ((threadIdx.x & 63) == 0)
-          epart[blk * (BT / 64) + (threadIdx.x >> 6)] = eacc;
+          epart[blk * (blockDim.x >> 6) + (threadIdx.x >> 6)] = eacc;
       }
     }
     if (live)
@@ -1276,7 +1277,7 @@ __launch_bounds__((BW > 0                ? 64 * BW
       zbox[i] = T(0);
     __syncthreads();
     apply_body<T, N, Q, OP, REF, GATHER, BW, EO>(
-        tab, x, y, G, detJw, dofmap, num_cells, ba, 0, nullptr, zbox, zcell0,
+        tab, x, y, G, detJw, dofmap, num_cells, ba, seg, epart, zbox, zcell0,
         ncell);
     __syncthreads(); // every wave's box sums are in
     const unsigned ci = line / (unsigned)ba.ncy;
@@ -1562,6 +1563,12 @@ int launch_zrun(const ApplyReq& r)
   else
   {
     constexpr int S = zrun_cells<T, N, Q, OP>();
+    if (r.zrun_shape)
+    {
+      r.zrun_shape[0] = S;
+      r.zrun_shape[1] = zrun_waves<T, N, Q, OP>();
+      return FEMTC_OK;
+    }
     if (r.ncx < 1 || r.ncy < 1 || r.ncz < 1 ||
         (long long)r.ncx * r.ncy * r.ncz != r.num_cells ||
         r.num_cells > 0x7fffffffLL)
@@ -1584,7 +1591,7 @@ int launch_zrun(const ApplyReq& r)
                        dim3((unsigned)grid), dim3(64 * W), lds, r.stream,
                        (const T*)r.tab, (const T*)r.x, (T*)r.y, (const T*)r.G,
                        (const T*)r.detJw, r.dofmap, (long long)r.num_cells,
-                       r.skip, ba, nullptr);
+                       r.skip, ba, r.energy);
     return hip_status(hipGetLastError());
   }
 }
@@ -1643,6 +1650,9 @@ int dispatch_op(const ApplyReq& r)
     case OP_MASS: return launch_zrun<T, N, Q, OP_MASS>(r);
     case OP_STIFF: return launch_zrun<T, N, Q, OP_STIFF>(r);
     case OP_KPM: return launch_zrun<T, N, Q, OP_KPM>(r);
+    case OP_MASS_AFF: return launch_zrun<T, N, Q, OP_MASS_AFF>(r);
+    case OP_STIFF_AFF: return launch_zrun<T, N, Q, OP_STIFF_AFF>(r);
+    case OP_KPM_AFF: return launch_zrun<T, N, Q, OP_KPM_AFF>(r);
     }
     return FEMTC_ERR_INVALID_ARGUMENT;
   }

@@ -78,6 +78,9 @@ struct ApplyReq
   // workgroup sums a segment of cells along z in LDS and adds the segment's
   // dof box to y with atomics in runs of consecutive global dofs.
   int zrun = 0;
+  // z-run scatter only: when set, the launcher stores the instance's
+  // (cells per segment, waves per workgroup) here and launches nothing
+  int* zrun_shape = nullptr;
   // Even-odd table set (even_odd.hpp) of symmetric tables, used instead of
   // tab when set: Q == N, gather, correct mode, direct or assembled scatter.
   const void* eo_tab = nullptr;

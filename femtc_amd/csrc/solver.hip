@@ -639,11 +639,15 @@ struct femtc_solver
   double* cg64 = nullptr;   // [C][7], FP64
   double* epart = nullptr;  // per-wave p.Ap partials of the inner operator
   int64_t nepart = 0;
+  int64_t nepart_cap = 0;   // partials the epart allocation holds
   int check_every = 16;
   // assembled scatter of the inner operator (problem.grid, structured
   // dofmap): its lattice buffer in the inner precision
   bool inner_assembled = false;
   void* lattice = nullptr;
+  // structured z-run scatter of the inner operator (the atomic scatter's
+  // Ap and k_update)
+  bool inner_zrun = false;
   // even-odd table sets (symmetric tables, nq = degree + 1): FP64 for the
   // FP64 operators, inner precision for the inner one (aliases eo64 for an
   // FP64 inner solve)
@@ -783,7 +787,34 @@ ApplyReq inner_req(const femtc_solver* S, int dtype, const void* x, void* y)
     a.ncy = S->prob.grid[1];
     a.ncz = S->prob.grid[2];
   }
+  else if (S->inner_zrun)
+  {
+    a.zrun = 1;
+    a.ncx = S->prob.grid[0];
+    a.ncy = S->prob.grid[1];
+    a.ncz = S->prob.grid[2];
+  }
   return a;
+}
+
+// Energy partials of the inner operator's z-run launch in precision dtype:
+// one per wave of each z-segment's workgroup.
+int zrun_partials(const femtc_solver* S, int dtype, int64_t* npart)
+{
+  ApplyReq a = operator_req(S, dtype, S->tab, S->G, S->detJw, nullptr,
+                            nullptr);
+  if (S->inner_affine)
+    a.op += OP_MASS_AFF;
+  a.eo_tab = nullptr;
+  a.zrun = 1;
+  int shape[2] = {0, 0};
+  a.zrun_shape = shape;
+  const int st = run(a);
+  if (st != FEMTC_OK)
+    return st;
+  const int64_t nzs = (S->prob.grid[2] + shape[0] - 1) / shape[0];
+  *npart = (int64_t)S->prob.grid[0] * S->prob.grid[1] * nzs * shape[1];
+  return FEMTC_OK;
 }
 
 ApplyReq outer_req(const femtc_solver* S, const void* x, void* y)
@@ -921,11 +952,12 @@ int64_t energy_partials(int64_t C, int Q, bool assembled, int dtype)
 }
 
 // Scatter and contraction form of the operators.  The scatters allowed by
-// problem.grid/scatter (the assembled one only for the structured dofmap;
-// FEMTC_SCATTER_AUTO allows both) and the forms available (even-odd when
-// the tables passed the check) are each timed over one CG iteration step
-// on this problem, and the fastest pair is kept (vectors zeroed and
-// alpha = 0 for the timing; the solve re-initialises everything).
+// problem.grid/scatter (the assembled and z-run ones only for the
+// structured dofmap; FEMTC_SCATTER_AUTO allows all three) and the forms
+// available (even-odd when the tables passed the check, direct and
+// assembled scatters) are each timed over one CG iteration step on this
+// problem, and the fastest pair is kept (vectors zeroed and alpha = 0 for
+// the timing; the solve re-initialises everything).
 int choose_variant(femtc_solver* S, int structured)
 {
   if (S->mfma_split >= 0)
@@ -935,6 +967,10 @@ int choose_variant(femtc_solver* S, int structured)
       return fail(FEMTC_ERR_INVALID_ARGUMENT,
                   "BF16 inner operator: the dofmap is not the structured "
                   "numbering of the grid");
+    if (S->prob.scatter == FEMTC_SCATTER_DIRECT ||
+        S->prob.scatter == FEMTC_SCATTER_ZRUN)
+      return fail(FEMTC_ERR_INVALID_ARGUMENT,
+                  "BF16 inner operator: assembled scatter only");
     S->inner_assembled = true;
     S->even_odd = false;
     S->nepart = S->prob.num_cells; // one energy partial per cell (wave)
@@ -944,7 +980,11 @@ int choose_variant(femtc_solver* S, int structured)
   }
   const int mode = S->prob.scatter;
   const bool have_eo = S->eo64 && S->eo_in;
-  bool try_asm = structured && mode != FEMTC_SCATTER_DIRECT;
+  bool try_asm = structured && (mode == FEMTC_SCATTER_AUTO ||
+                                mode == FEMTC_SCATTER_ASSEMBLED);
+  const bool try_zrun = structured && S->Q <= 8 &&
+                        (mode == FEMTC_SCATTER_AUTO ||
+                         mode == FEMTC_SCATTER_ZRUN);
   if (try_asm)
   {
     const size_t es = S->inner_dtype == FEMTC_F64 ? 8 : 4;
@@ -957,16 +997,30 @@ int choose_variant(femtc_solver* S, int structured)
       try_asm = false; // no room: keep the atomic scatter
     }
   }
-  const bool try_direct = !(try_asm && mode == FEMTC_SCATTER_ASSEMBLED);
+  const bool try_direct =
+      mode == FEMTC_SCATTER_DIRECT || mode == FEMTC_SCATTER_AUTO ||
+      (mode == FEMTC_SCATTER_ASSEMBLED && !try_asm);
   // even-odd forms to try: none, the even-odd one only (FEMTC_EVEN_ODD_ON),
-  // or both
+  // or both; the z-run scatter has no even-odd form
   const int e0 = have_eo && S->prob.even_odd == FEMTC_EVEN_ODD_ON ? 1 : 0;
   const int e1 = have_eo ? 2 : 1;
-  std::vector<std::pair<bool, bool>> cand; // (assembled, even-odd)
-  for (int a = 0; a < 2; ++a)
-    if (a ? try_asm : try_direct)
+  // (scatter: 0 direct, 1 assembled, 2 z-run; even-odd)
+  std::vector<std::pair<int, bool>> cand;
+  for (int a = 0; a < 3; ++a)
+    if (a == 0 ? try_direct : a == 1 ? try_asm : try_zrun)
+    {
+      if (a == 2)
+      {
+        if (e0 == 0)
+          cand.push_back({2, false});
+        continue;
+      }
       for (int e = e0; e < e1; ++e)
-        cand.push_back({a == 1, e == 1});
+        cand.push_back({a, e == 1});
+    }
+  if (cand.empty())
+    return fail(FEMTC_ERR_INVALID_ARGUMENT,
+                "the z-run scatter has no even-odd form");
   size_t best = 0;
   int st = FEMTC_OK;
   if (cand.size() > 1)
@@ -985,7 +1039,8 @@ int choose_variant(femtc_solver* S, int structured)
     float tbest = 0.f;
     for (size_t c = 0; c < cand.size() && st == FEMTC_OK; ++c)
     {
-      S->inner_assembled = cand[c].first;
+      S->inner_assembled = cand[c].first == 1;
+      S->inner_zrun = cand[c].first == 2;
       S->even_odd = cand[c].second;
       float t = 0.f;
       st = dt == FEMTC_F64 ? time_step<double>(S, 3, &t)
@@ -997,10 +1052,23 @@ int choose_variant(femtc_solver* S, int structured)
       }
     }
   }
-  S->inner_assembled = cand[best].first;
+  S->inner_assembled = cand[best].first == 1;
+  S->inner_zrun = cand[best].first == 2;
   S->even_odd = cand[best].second;
   S->nepart = energy_partials(S->prob.num_cells, S->Q, S->inner_assembled,
                               S->inner_dtype);
+  if (S->inner_zrun && st == FEMTC_OK)
+    st = zrun_partials(S, S->inner_dtype, &S->nepart);
+  if (st == FEMTC_OK && S->nepart > S->nepart_cap)
+  {
+    // the z-run launch has a wave per segment slot: more partials
+    dev_free(S->epart);
+    S->bytes -= 8 * (size_t)S->nepart_cap;
+    S->epart = nullptr;
+    S->nepart_cap = 0;
+    if ((st = dev_alloc(S, (void**)&S->epart, 8 * S->nepart)) == FEMTC_OK)
+      S->nepart_cap = S->nepart;
+  }
   if (!S->inner_assembled && S->lattice)
   {
     dev_free(S->lattice);
@@ -1201,11 +1269,15 @@ int check_problem(const femtc_problem* P)
     return fail(FEMTC_ERR_INVALID_ARGUMENT,
                 "grid must be all zero or positive with grid[0]*grid[1]*grid[2] "
                 "== num_cells");
-  if (P->scatter < FEMTC_SCATTER_AUTO || P->scatter > FEMTC_SCATTER_ASSEMBLED)
+  if (P->scatter < FEMTC_SCATTER_AUTO || P->scatter > FEMTC_SCATTER_ZRUN)
     return fail(FEMTC_ERR_INVALID_ARGUMENT, "scatter must be a femtc_scatter");
-  if (P->scatter == FEMTC_SCATTER_ASSEMBLED && nogrid)
+  if ((P->scatter == FEMTC_SCATTER_ASSEMBLED ||
+       P->scatter == FEMTC_SCATTER_ZRUN) &&
+      nogrid)
     return fail(FEMTC_ERR_INVALID_ARGUMENT,
-                "the assembled scatter needs the structured grid");
+                "the assembled and z-run scatters need the structured grid");
+  if (P->scatter == FEMTC_SCATTER_ZRUN && P->nq > 8)
+    return fail(FEMTC_ERR_INVALID_ARGUMENT, "the z-run scatter needs nq <= 8");
   if (P->even_odd < FEMTC_EVEN_ODD_AUTO || P->even_odd > FEMTC_EVEN_ODD_ON)
     return fail(FEMTC_ERR_INVALID_ARGUMENT,
                 "even_odd must be FEMTC_EVEN_ODD_AUTO, _OFF or _ON");
@@ -1297,6 +1369,7 @@ int femtc_solver_create(femtc_solver** out, const femtc_problem* prob,
                               energy_partials(C, S->Q, true, inner_dtype));
     if ((st = dev_alloc(S, (void**)&S->epart, 8 * S->nepart)))
       break;
+    S->nepart_cap = S->nepart;
     if (hipHostMalloc((void**)&S->hst, sizeof(PcgState)) != hipSuccess ||
         hipHostMalloc((void**)&S->hscal, sizeof(double)) != hipSuccess)
     {
@@ -1462,11 +1535,13 @@ int femtc_solver_create(femtc_solver** out, const femtc_problem* prob,
                                 prob->grid[2], prob->degree, n, s,
                                 &structured)) != FEMTC_OK)
       break;
-    if (prob->scatter == FEMTC_SCATTER_ASSEMBLED && !structured)
+    if ((prob->scatter == FEMTC_SCATTER_ASSEMBLED ||
+         prob->scatter == FEMTC_SCATTER_ZRUN) &&
+        !structured)
     {
       st = fail(FEMTC_ERR_INVALID_ARGUMENT,
-                "assembled scatter: the dofmap is not the structured "
-                "numbering of the grid");
+                "assembled or z-run scatter: the dofmap is not the "
+                "structured numbering of the grid");
       break;
     }
     if ((st = choose_variant(S, structured)) != FEMTC_OK)
@@ -1530,7 +1605,8 @@ int femtc_solver_scatter(const femtc_solver* solver, int* scatter)
   if (!solver || !scatter)
     return fail(FEMTC_ERR_INVALID_ARGUMENT, "NULL argument");
   *scatter = solver->inner_assembled ? FEMTC_SCATTER_ASSEMBLED
-                                     : FEMTC_SCATTER_DIRECT;
+             : solver->inner_zrun       ? FEMTC_SCATTER_ZRUN
+                                        : FEMTC_SCATTER_DIRECT;
   return FEMTC_OK;
 }
 

@@ -41,8 +41,9 @@ class PCGSolver:
         inner operator then computes its geometry in the kernel.
         grid: optional (nx, ny, nz) of a structured cell numbering; when the
         dofmap is that numbering the CG iterations can use the assembled
-        scatter (scatter: SCATTER_AUTO times it against the atomic one,
-        SCATTER_DIRECT / SCATTER_ASSEMBLED force one).
+        or the z-run scatter (scatter: SCATTER_AUTO times them against the
+        atomic one, SCATTER_DIRECT / SCATTER_ASSEMBLED / SCATTER_ZRUN force
+        one).
         even_odd: None times the even-odd form (nq = degree + 1 and
         symmetric tables, checked once, here) against the general one and
         keeps the faster; True uses it whenever the check passes; False
@@ -70,8 +71,8 @@ class PCGSolver:
                    "solver_set_batch")
 
     def scatter(self) -> int:
-        """Scatter of the CG iterations chosen at creation: SCATTER_DIRECT or
-        SCATTER_ASSEMBLED."""
+        """Scatter of the CG iterations chosen at creation: SCATTER_DIRECT,
+        SCATTER_ASSEMBLED or SCATTER_ZRUN."""
         a = C.c_int32()
         _lib.check(_lib.lib().femtc_solver_scatter(self._h, C.byref(a)),
                    "solver_scatter")

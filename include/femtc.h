@@ -490,9 +490,13 @@ typedef enum femtc_scatter
   FEMTC_SCATTER_AUTO = 0,     /* time one iteration step with each (when the
                                  dofmap allows it), keep the faster */
   FEMTC_SCATTER_DIRECT = 1,   /* atomic scatter */
-  FEMTC_SCATTER_ASSEMBLED = 2 /* assembled scatter; creation fails with
-                                 FEMTC_ERR_INVALID_ARGUMENT if the dofmap
-                                 is not the structured one */
+  FEMTC_SCATTER_ASSEMBLED = 2, /* assembled scatter; creation fails with
+                                  FEMTC_ERR_INVALID_ARGUMENT if the dofmap
+                                  is not the structured one */
+  FEMTC_SCATTER_ZRUN = 3      /* structured z-run scatter (cells summed
+                                 along z in LDS, atomics on runs of
+                                 consecutive dofs; nq <= 8); fails as the
+                                 assembled one */
 } femtc_scatter;
 
 typedef struct femtc_solve_info
@@ -537,7 +541,7 @@ int femtc_solver_solve(femtc_solver* solver, const double* f, double* u,
 /* Device pointer to the solver's FP64 operator diagonal. */
 int femtc_solver_diagonal(const femtc_solver* solver, const double** diag);
 /* The scatter the solver chose for its CG iterations
- * (FEMTC_SCATTER_DIRECT or FEMTC_SCATTER_ASSEMBLED). */
+ * (FEMTC_SCATTER_DIRECT, FEMTC_SCATTER_ASSEMBLED or FEMTC_SCATTER_ZRUN). */
 int femtc_solver_scatter(const femtc_solver* solver, int* scatter);
 /* 1 when the solver's operators contract in even-odd form, else 0. */
 int femtc_solver_even_odd(const femtc_solver* solver, int* even_odd);

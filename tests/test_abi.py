@@ -156,7 +156,17 @@ def test_solver_create_rejects_bad_problems_without_touching_the_device():
                      F.SCATTER_ASSEMBLED)
     assert L.femtc_solver_create(C_.byref(h), C_.byref(p), F.F64, None) == \
         _lib.ERR_INVALID_ARGUMENT
-    assert "needs the structured grid" in _lib.last_error()
+    assert "need the structured grid" in _lib.last_error()
+    p = _lib.Problem(F.OP_KPM, 3, 4, 10, 100, 1, 1, 1, 1, None, grid(0, 0, 0),
+                     F.SCATTER_ZRUN)
+    assert L.femtc_solver_create(C_.byref(h), C_.byref(p), F.F64, None) == \
+        _lib.ERR_INVALID_ARGUMENT
+    assert "need the structured grid" in _lib.last_error()
+    p = _lib.Problem(F.OP_KPM, 8, 9, 10, 100, 1, 1, 1, 1, None, grid(1, 2, 5),
+                     F.SCATTER_ZRUN)
+    assert L.femtc_solver_create(C_.byref(h), C_.byref(p), F.F64, None) == \
+        _lib.ERR_INVALID_ARGUMENT
+    assert "nq <= 8" in _lib.last_error()
     p = _lib.Problem(F.OP_KPM, 3, 4, 10, 100, 1, 1, 1, 1, None, grid(1, 2, 5),
                      7)
     assert L.femtc_solver_create(C_.byref(h), C_.byref(p), F.F64, None) == \

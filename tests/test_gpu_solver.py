@@ -162,7 +162,10 @@ def test_solver_assembled_scatter(inner, op, nx, ny, nz, P, Q, affine):
     if affine:
         kw["J"] = dev(O.random_jacobians(seed, C))
     iters = {}
-    for scatter in (F.SCATTER_DIRECT, F.SCATTER_ASSEMBLED, F.SCATTER_AUTO):
+    scatters = [F.SCATTER_DIRECT, F.SCATTER_ASSEMBLED, F.SCATTER_AUTO]
+    if Q <= 8:
+        scatters.insert(2, F.SCATTER_ZRUN)
+    for scatter in scatters:
         with F.PCGSolver(op, dev(tab), dev(dm, torch.int32), C, P, Q, ng,
                          inner=inner, grid=(nx, ny, nz), scatter=scatter,
                          **kw) as S:
@@ -184,6 +187,9 @@ def test_solver_assembled_scatter(inner, op, nx, ny, nz, P, Q, affine):
     # from run to run (the atomic sums' last bits)
     a, b = iters[F.SCATTER_DIRECT], iters[F.SCATTER_ASSEMBLED]
     assert abs(a - b) <= max(2, 0.02 * max(a, b)), iters
+    if F.SCATTER_ZRUN in iters:
+        c = iters[F.SCATTER_ZRUN]
+        assert abs(a - c) <= max(2, 0.02 * max(a, c)), iters
 
 
 @pytest.mark.parametrize("inner", [F.F64, F.F32])
@@ -231,6 +237,9 @@ def test_solver_assembled_scatter_needs_the_structured_dofmap():
     with pytest.raises(F.FemtcError, match="structured"):
         F.PCGSolver(*args, G=dev(G), detJw=dev(Dw), grid=(nx, ny, nz),
                     scatter=F.SCATTER_ASSEMBLED)
+    with pytest.raises(F.FemtcError, match="structured"):
+        F.PCGSolver(*args, G=dev(G), detJw=dev(Dw), grid=(nx, ny, nz),
+                    scatter=F.SCATTER_ZRUN)
     # the automatic choice keeps the atomic scatter and solves correctly
     with F.PCGSolver(*args, G=dev(G), detJw=dev(Dw), grid=(nx, ny, nz)) as S:
         assert S.scatter() == F.SCATTER_DIRECT
@@ -240,6 +249,23 @@ def test_solver_assembled_scatter_needs_the_structured_dofmap():
     assert info["relres"] <= 1.01e-10 and res <= 1.05e-10
     with pytest.raises(F.FemtcError, match="grid"):
         F.PCGSolver(*args, G=dev(G), detJw=dev(Dw), grid=(nx, ny, nz + 1))
+
+
+@pytest.mark.parametrize("inner", [F.F64, F.F32])
+def test_solver_zrun_scatter_has_no_even_odd_form(inner):
+    op, nx, ny, nz, P, Q = F.OP_KPM, 2, 3, 5, 3, 4
+    C, G, Dw, dm, ng, tab, f = _problem(op, nx, ny, nz, P, Q, 41)
+    args = (op, dev(tab), dev(dm, torch.int32), C, P, Q, ng)
+    kw = dict(G=dev(G), detJw=dev(Dw), grid=(nx, ny, nz), inner=inner,
+              scatter=F.SCATTER_ZRUN)
+    with pytest.raises(F.FemtcError, match="even-odd"):
+        F.PCGSolver(*args, even_odd=True, **kw)
+    with F.PCGSolver(*args, **kw) as S:
+        assert S.scatter() == F.SCATTER_ZRUN and S.even_odd() is False
+        u = torch.empty(ng, device="cuda", dtype=torch.float64)
+        info = S.solve(dev(f), u, rtol=1e-10)
+    res = rel(O.global_operator("kpm", P + 1, Q, tab, host(u), dm, G, Dw), f)
+    assert info["relres"] <= 1.01e-10 and res <= 1.05e-10
 
 
 def test_solver_reuse_is_consistent():
