@@ -552,7 +552,7 @@ def bench_batch(torch, F):
         nbytes = sum(Wk.algorithmic_bytes(it, es) for it in items)
         b = F.Batch(items)
         st = b.stats()
-        b.autotune(3)
+        b.autotune(5)
         for _ in range(3):
             b.graph_launch()
         torch.cuda.synchronize()

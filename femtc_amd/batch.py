@@ -67,8 +67,8 @@ class Batch:
 
     def set_group(self, group: int, per_wg: int) -> None:
         """per_wg = 0: one launch per problem; k > 0: fused, k blocks per
-        workgroup; -1: the assembled scatter per problem (structured
-        items only)."""
+        workgroup; -1: the assembled scatter per problem, -2: the z-run
+        scatter per problem (structured items only; -2 also nq <= 8)."""
         _lib.check(_lib.lib().femtc_batch_set_group(self._h, group, per_wg),
                    "batch_set_group")
 

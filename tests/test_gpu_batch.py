@@ -136,28 +136,29 @@ def test_batch_autotune_keeps_results():
     b.graph_launch()  # a graph captured before tuning is dropped by it
     b.autotune(reps=2)
     for g in b.groups():
-        assert g["per_wg"] in (0, 1, 2, 4, 8) or (g["per_wg"] == -1 and
+        assert g["per_wg"] in (0, 1, 2, 4, 8) or (g["per_wg"] in (-1, -2) and
                                                   g["gather"] == 1)
     b.graph_launch()
     torch.cuda.synchronize()
     _check(items, 1e-12)
 
 
+@pytest.mark.parametrize("scatter", [-1, -2])
 @pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-12),
                                        (torch.float32, 1e-5)])
-def test_batch_assembled_scatter_vs_alone_and_oracle(dtype, tol):
-    # continuous groups on the assembled scatter (every item carries its
-    # verified grid), cell-local groups as they are; eager, graph replay,
-    # and a graph replay after eager work
+def test_batch_structured_scatters_vs_alone_and_oracle(dtype, tol, scatter):
+    # continuous groups on the assembled (-1) or z-run (-2) scatter (every
+    # item carries its verified grid), cell-local groups as they are; eager,
+    # graph replay, and a graph replay after eager work
     items = _items(dtype, seed0=800)
     b = F.Batch(items)
     for g, info in enumerate(b.groups()):
-        if info["gather"]:
-            b.set_group(g, -1)
-            assert b.groups()[g]["per_wg"] == -1
+        if info["gather"] and (scatter == -1 or info["nq"] <= 8):
+            b.set_group(g, scatter)
+            assert b.groups()[g]["per_wg"] == scatter
         else:
             with pytest.raises(F.FemtcError, match="structured grid"):
-                b.set_group(g, -1)
+                b.set_group(g, scatter)
     for it in items:
         it["y"].fill_(7.0)
     b.run()
@@ -185,8 +186,10 @@ def test_batch_assembled_scatter_needs_structured_items():
     b.set_group(ops[items[0]["op"]], -1)
     with pytest.raises(F.FemtcError, match="structured grid"):
         b.set_group(ops[it["op"]], -1)
+    with pytest.raises(F.FemtcError, match="structured grid"):
+        b.set_group(ops[it["op"]], -2)
     b.autotune(reps=1)
-    assert b.groups()[ops[it["op"]]]["per_wg"] != -1
+    assert b.groups()[ops[it["op"]]]["per_wg"] not in (-1, -2)
     b.run()
     _check(items, 1e-12)
 
