@@ -416,11 +416,27 @@ def plan_scatters(torch, F, pr, x, y, steps=30, warmup=60):
     shape, _ = plan.autotune(F.OP_STIFFNESS, pr["tab"], x, y, Q, G=pr["G"],
                              reps=5)
     pms = timed()
+    # the chosen scatter's operator kernel alone: accumulating applies
+    # (y += Kx) need no clear of y
+    for _ in range(warmup):
+        plan.apply(F.OP_STIFFNESS, pr["tab"], x, y, Q, G=pr["G"],
+                   accumulate=True)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        plan.apply(F.OP_STIFFNESS, pr["tab"], x, y, Q, G=pr["G"],
+                   accumulate=True)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    kms = e0.elapsed_time(e1) / steps
     plan.close()
     return {"scatter": names.get(tuple(shape), f"blocked {tuple(shape)}"),
             "ms_per_apply": pms,
             "gdofs": pr["ng"] / (pms * 1e-3) / 1e9,
             "roofline_frac": ab / (pms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "kernel_ms": kms,
+            "kernel_roofline_frac": ab / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "by_scatter": by_scatter}
 
 
