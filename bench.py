@@ -70,11 +70,13 @@ def measured_traffic():
 
 
 def cpu_baseline(P: int, ncells_global: int, ndofs_global: int,
-                 budget_s: float = 12.0) -> dict:
+                 budget_s: float = 15.0) -> dict:
     """Reference stiffness kernel on the host cores, one thread per core on
-    disjoint cell ranges (ctypes releases the GIL), 10 applies, mean of the
-    last 5 (bench_stiffness.cpp:95-106).  Reported as global-dof throughput
-    of the same mesh: cells/s * (ndofs_global / ncells_global)."""
+    disjoint cell ranges (ctypes releases the GIL): applies repeated until
+    about budget_s of CPU time (at least 10), mean of the second half
+    (bench_stiffness.cpp:95-106 times 10 and keeps the last 5).  Reported as
+    global-dof throughput of the same mesh: cells/s * (ndofs_global /
+    ncells_global)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O  # checker / baseline infrastructure only
 
@@ -106,8 +108,8 @@ def cpu_baseline(P: int, ncells_global: int, ndofs_global: int,
                 best = (dt, order, mb, nb)
     _, order, mb, nb = best
     per_cell = best[0] / probe
-    # the whole mesh split evenly over the threads, unless that would exceed
-    # about budget_s of CPU time over the 10 applies
+    # the whole mesh split evenly over the threads, unless 10 applies would
+    # exceed about budget_s of CPU time
     per_thread = -(-ncells_global // cores)
     per_thread = int(max(64, min(per_thread, budget_s / (10 * per_cell))))
     bufs = []
@@ -116,7 +118,7 @@ def cpu_baseline(P: int, ncells_global: int, ndofs_global: int,
                      np.zeros(per_thread * nd),
                      _uniform(50 + t, per_thread * nd * 6, 0, 1)))
     times = []
-    for it in range(10):
+    while len(times) < 10 or sum(times) * cores < budget_s:
         barrier = threading.Barrier(cores + 1)
         done = threading.Barrier(cores + 1)
 
@@ -135,7 +137,7 @@ def cpu_baseline(P: int, ncells_global: int, ndofs_global: int,
         times.append(time.perf_counter() - t0)
         for th in ths:
             th.join()
-    elapsed = float(np.mean(times[5:]))
+    elapsed = float(np.mean(times[len(times) // 2:]))
     cells_per_s = cores * per_thread / elapsed
     names = ("ijk", "ikj", "jik", "jki", "kij", "kji")
     return {
@@ -146,7 +148,9 @@ def cpu_baseline(P: int, ncells_global: int, ndofs_global: int,
         "sample": (f"reference stiffness_operator<double,P={P}> "
                    f"(src/stiffness.hpp:34-99, -Ofast {variant}), "
                    f"{cores} threads x {per_thread} cells, order={names[order]} "
-                   f"Mb={mb} Nb={nb}, mean of applies 5-9 of 10 = "
+                   f"Mb={mb} Nb={nb}, {len(times)} applies "
+                   f"({sum(times) * cores:.1f} s of CPU time), mean of the "
+                   f"last {len(times) - len(times) // 2} = "
                    f"{elapsed * 1e3:.1f} ms; {cells_per_s:.4g} cells/s scaled "
                    f"to the mesh's global dofs per cell"),
         "cell_local_gdofs": cells_per_s * nd / 1e9,
