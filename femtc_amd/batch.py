@@ -68,7 +68,8 @@ class Batch:
     def set_group(self, group: int, per_wg: int) -> None:
         """per_wg = 0: one launch per problem; k > 0: fused, k blocks per
         workgroup; -1: the assembled scatter per problem, -2: the z-run
-        scatter per problem (structured items only; -2 also nq <= 8)."""
+        scatter per problem, -3: one fused z-run launch over the group
+        (structured items only; -2 and -3 also nq <= 8)."""
         _lib.check(_lib.lib().femtc_batch_set_group(self._h, group, per_wg),
                    "batch_set_group")
 

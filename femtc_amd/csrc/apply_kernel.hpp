@@ -1220,6 +1220,59 @@ constexpr int min_waves_per_simd()
              : 1;
 }
 
+// The z-run scatter's work on segment seg of a structured ncx x ncy x ncz
+// grid (ba): zrun_cells cells of one z-line (fewer at its end), segments
+// numbered (ci*ncy + cj)*nzs + zs.  Each wave takes CPB cells of the segment
+// in its own LDS work area; their values are summed into the (a, b, z) box
+// of the segment's dofs in LDS, and the box is then added to y with atomics,
+// consecutive lanes on consecutive global dofs along z.  Workgroups of
+// 64 * zrun_waves threads with the dynamic LDS of launch_zrun.
+template <typename T, int N, int Q, int OP>
+__device__ __forceinline__ void
+zrun_segment(const T* __restrict__ tab, const T* __restrict__ x,
+             T* __restrict__ y, const T* __restrict__ G,
+             const T* __restrict__ detJw, long long num_cells,
+             const BlockArgs& ba, long long seg, double* __restrict__ epart)
+{
+  constexpr int P = N - 1;
+  constexpr int S = zrun_cells<T, N, Q, OP>();
+  constexpr int ZB = S * P + 1;
+  constexpr int BOXN = N * N * ZB;
+  constexpr int BT = 64 * zrun_waves<T, N, Q, OP>();
+  extern __shared__ __align__(16) unsigned char femtc_lds_dyn[];
+  T* zbox = reinterpret_cast<T*>(femtc_lds_dyn +
+                                 zrun_waves<T, N, Q, OP>() *
+                                     blocked_work_bytes<T, N, Q, OP>());
+  const unsigned nzs = ((unsigned)ba.ncz + S - 1) / S;
+  const unsigned line = (unsigned)seg / nzs; // ci*ncy + cj
+  const int ck0 = (int)((unsigned)seg - line * nzs) * S;
+  const int ncell = ba.ncz - ck0 < S ? ba.ncz - ck0 : S;
+  const long long zcell0 = (long long)line * ba.ncz + ck0;
+  for (int i = threadIdx.x; i < BOXN; i += BT)
+    zbox[i] = T(0);
+  __syncthreads();
+  apply_body<T, N, Q, OP, false, true, SCATTER_ZRUN, false>(
+      tab, x, y, G, detJw, nullptr, num_cells, ba, seg, epart, zbox, zcell0,
+      ncell);
+  __syncthreads(); // every wave's box sums are in
+  const unsigned ci = line / (unsigned)ba.ncy;
+  const unsigned cj = line - ci * (unsigned)ba.ncy;
+  const long long NYg = (long long)ba.ncy * P + 1;
+  const long long NZg = (long long)ba.ncz * P + 1;
+  const int zlen = ncell * P + 1;
+  const long long gz0 = (long long)ck0 * P;
+  for (int i = threadIdx.x; i < BOXN; i += BT)
+  {
+    const int ab = i / ZB, z = i - ab * ZB;
+    if (z < zlen)
+    {
+      const int a = ab / N, b = ab - a * N;
+      atomic_add(y + ((ci * P + a) * NYg + cj * P + b) * NZg + gz0 + z,
+                 zbox[i]);
+    }
+  }
+}
+
 template <typename T, int N, int Q, int OP, bool REF, bool GATHER, int BW,
           bool EO = false>
 __global__ void
@@ -1248,54 +1301,13 @@ __launch_bounds__((BW > 0                ? 64 * BW
   }
   if constexpr (BW == SCATTER_ZRUN)
   {
-    // One z-segment of zrun_cells cells (fewer at the end of a z-line) per
-    // workgroup, segments numbered (ci*ncy + cj)*nzs + zs and dealt to the
-    // XCDs in contiguous ranges as below.  Each wave takes CPB cells of the
-    // segment in its own LDS work area; their values are summed into the
-    // (a, b, z) box of the segment's dofs in LDS, and the box is then added
-    // to y with atomics, consecutive lanes on consecutive global dofs
-    // along z.
-    constexpr int P = N - 1;
-    constexpr int S = zrun_cells<T, N, Q, OP>();
-    constexpr int ZB = S * P + 1;
-    constexpr int BOXN = N * N * ZB;
-    constexpr int BT = 64 * zrun_waves<T, N, Q, OP>();
+    // one z-segment per workgroup, segments dealt to the XCDs in contiguous
+    // ranges as below
     const long long per = (ba.nblk + 7) / 8;
     const long long seg = (long long)(blockIdx.x % 8) * per + blockIdx.x / 8;
     if (seg >= ba.nblk)
       return; // uniform over the workgroup
-    extern __shared__ __align__(16) unsigned char femtc_lds_dyn[];
-    T* zbox = reinterpret_cast<T*>(femtc_lds_dyn +
-                                   zrun_waves<T, N, Q, OP>() *
-                                       blocked_work_bytes<T, N, Q, OP>());
-    const unsigned nzs = ((unsigned)ba.ncz + S - 1) / S;
-    const unsigned line = (unsigned)seg / nzs; // ci*ncy + cj
-    const int ck0 = (int)((unsigned)seg - line * nzs) * S;
-    const int ncell = ba.ncz - ck0 < S ? ba.ncz - ck0 : S;
-    const long long zcell0 = (long long)line * ba.ncz + ck0;
-    for (int i = threadIdx.x; i < BOXN; i += BT)
-      zbox[i] = T(0);
-    __syncthreads();
-    apply_body<T, N, Q, OP, REF, GATHER, BW, EO>(
-        tab, x, y, G, detJw, dofmap, num_cells, ba, seg, epart, zbox, zcell0,
-        ncell);
-    __syncthreads(); // every wave's box sums are in
-    const unsigned ci = line / (unsigned)ba.ncy;
-    const unsigned cj = line - ci * (unsigned)ba.ncy;
-    const long long NYg = (long long)ba.ncy * P + 1;
-    const long long NZg = (long long)ba.ncz * P + 1;
-    const int zlen = ncell * P + 1;
-    const long long gz0 = (long long)ck0 * P;
-    for (int i = threadIdx.x; i < BOXN; i += BT)
-    {
-      const int ab = i / ZB, z = i - ab * ZB;
-      if (z < zlen)
-      {
-        const int a = ab / N, b = ab - a * N;
-        atomic_add(y + ((ci * P + a) * NYg + cj * P + b) * NZg + gz0 + z,
-                   zbox[i]);
-      }
-    }
+    zrun_segment<T, N, Q, OP>(tab, x, y, G, detJw, num_cells, ba, seg, epart);
     return;
   }
   if constexpr (BW == SCATTER_ASSEMBLED || BW == SCATTER_STRUCT)
@@ -1353,6 +1365,72 @@ __global__ void __launch_bounds__(block_threads<Q>())
   }
 }
 
+// Batched z-run launch: the z-segments of many structured problems of one
+// (T, N, Q, OP) in one grid, problem i owning segments [blk_begin_i,
+// blk_begin_{i+1}), segments dealt to the XCDs in contiguous ranges.
+template <typename T, int N, int Q, int OP>
+__global__ void __launch_bounds__((64 * zrun_waves<T, N, Q, OP>()))
+    apply_batch_zrun(const BatchDesc* __restrict__ items, int nitems,
+                     long long nseg_total)
+{
+  __shared__ long long first_seg[BATCH_MAX_ITEMS];
+  for (int i = threadIdx.x; i < nitems; i += blockDim.x)
+    first_seg[i] = items[i].blk_begin;
+  __syncthreads();
+  const long long per = (nseg_total + 7) / 8;
+  const long long seg = (long long)(blockIdx.x % 8) * per + blockIdx.x / 8;
+  if (seg >= nseg_total)
+    return; // uniform over the workgroup
+  int lo = 0, hi = nitems - 1;
+  while (lo < hi)
+  {
+    const int mid = (lo + hi + 1) >> 1;
+    if (first_seg[mid] <= seg)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const BatchDesc d = items[lo];
+  BlockArgs ba{};
+  ba.ncx = d.grid[0];
+  ba.ncy = d.grid[1];
+  ba.ncz = d.grid[2];
+  zrun_segment<T, N, Q, OP>((const T*)d.tab, (const T*)d.x, (T*)d.y,
+                            (const T*)d.G, (const T*)d.detJw, d.num_cells, ba,
+                            seg - d.blk_begin, nullptr);
+}
+
+template <typename T, int N, int Q, int OP>
+int launch_batch_zrun(const BatchReq& r)
+{
+  if constexpr (Q * Q > 64)
+    return FEMTC_ERR_INVALID_ARGUMENT;
+  else
+  {
+    constexpr int W = zrun_waves<T, N, Q, OP>();
+    if (r.zrun_shape)
+    {
+      r.zrun_shape[0] = zrun_cells<T, N, Q, OP>();
+      r.zrun_shape[1] = W;
+      return FEMTC_OK;
+    }
+    if (!r.gather || r.nitems < 1 || r.nitems > BATCH_MAX_ITEMS)
+      return FEMTC_ERR_INVALID_ARGUMENT;
+    if (r.nblk_total == 0)
+      return FEMTC_OK;
+    const long long grid = 8 * ((r.nblk_total + 7) / 8);
+    if (grid > 0x7fffffffLL)
+      return FEMTC_ERR_INVALID_ARGUMENT;
+    constexpr size_t lds = W * blocked_work_bytes<T, N, Q, OP>() +
+                           sizeof(T) * N * N *
+                               (zrun_cells<T, N, Q, OP>() * (N - 1) + 1);
+    hipLaunchKernelGGL((apply_batch_zrun<T, N, Q, OP>), dim3((unsigned)grid),
+                       dim3(64 * W), lds, r.stream, r.items, r.nitems,
+                       r.nblk_total);
+    return hip_status(hipGetLastError());
+  }
+}
+
 template <typename T, int N, int Q, int OP, bool GATHER>
 int launch_batch_t(const BatchReq& r)
 {
@@ -1371,6 +1449,8 @@ int launch_batch_t(const BatchReq& r)
 template <typename T, int N, int Q, int OP>
 int launch_batch_g(const BatchReq& r)
 {
+  if (r.zrun)
+    return launch_batch_zrun<T, N, Q, OP>(r);
   return r.gather ? launch_batch_t<T, N, Q, OP, true>(r)
                   : launch_batch_t<T, N, Q, OP, false>(r);
 }

@@ -83,8 +83,12 @@ struct Group
   bool structured = true; // every problem has a verified grid
   // 0: one fused apply_batch launch; 1: one launch per problem; 2: the
   // assembled scatter per problem; 3: the structured z-run scatter per
-  // problem
+  // problem; 4: one fused z-run launch over the group's problems
   int per_problem = 1;
+  // the fused z-run launch's descriptors (segment offsets, grids), built
+  // when that mode is first selected or timed
+  BatchDesc* zdev = nullptr;
+  long long zseg_total = 0;
 };
 
 // Lattice buffers of a group's problems (allocated once).
@@ -113,6 +117,52 @@ int ensure_lattices(Group& g)
   return FEMTC_OK;
 }
 
+// Descriptors of the fused z-run launch: per problem its grid and its
+// first z-segment (the segment length of the group's instance).
+int ensure_zrun(Group& g)
+{
+  if (g.zdev)
+    return FEMTC_OK;
+  if (!g.structured || !g.req.gather || g.req.Q > 8)
+    return fail(FEMTC_ERR_INVALID_ARGUMENT,
+                "z-run scatter: every problem of the group needs a verified "
+                "structured grid, and nq <= 8");
+  BatchReq q = g.req;
+  q.zrun = 1;
+  int shape[2] = {0, 0};
+  q.zrun_shape = shape;
+  BatchLauncher f = batch_launcher(q.N);
+  int st = f ? f(q) : FEMTC_ERR_DEGREE_NOT_SUPPORTED;
+  if (st != FEMTC_OK)
+    return fail(st, "z-run scatter: no instance for the group");
+  std::vector<BatchDesc> zh = g.host;
+  long long total = 0;
+  for (size_t i = 0; i < zh.size(); ++i)
+  {
+    const Extra& e = g.extra[i];
+    for (int k = 0; k < 3; ++k)
+      zh[i].grid[k] = e.grid[k];
+    zh[i].blk_begin = total;
+    total += (long long)e.grid[0] * e.grid[1] *
+             ((e.grid[2] + shape[0] - 1) / shape[0]);
+  }
+  if (hipMalloc((void**)&g.zdev, zh.size() * sizeof(BatchDesc)) != hipSuccess)
+  {
+    (void)hipGetLastError();
+    g.zdev = nullptr;
+    return fail(FEMTC_ERR_OUT_OF_MEMORY, "batch descriptor allocation failed");
+  }
+  if (hipMemcpy(g.zdev, zh.data(), zh.size() * sizeof(BatchDesc),
+                hipMemcpyHostToDevice) != hipSuccess)
+  {
+    (void)hipFree(g.zdev);
+    g.zdev = nullptr;
+    return fail(FEMTC_ERR_HIP, "batch descriptor upload failed");
+  }
+  g.zseg_total = total;
+  return FEMTC_OK;
+}
+
 void free_lattices(Group& g)
 {
   for (Extra& e : g.extra)
@@ -127,10 +177,18 @@ void free_lattices(Group& g)
 // z-run scatter).
 int launch_group(const Group& g, hipStream_t s)
 {
-  if (!g.per_problem)
+  if (!g.per_problem || g.per_problem == 4)
   {
     BatchReq r = g.req;
     r.stream = s;
+    if (g.per_problem == 4)
+    {
+      if (!g.zdev)
+        return FEMTC_ERR_INVALID_ARGUMENT;
+      r.zrun = 1;
+      r.items = g.zdev;
+      r.nblk_total = g.zseg_total;
+    }
     BatchLauncher f = batch_launcher(r.N);
     return f ? f(r) : FEMTC_ERR_DEGREE_NOT_SUPPORTED;
   }
@@ -225,7 +283,11 @@ void release(femtc_batch* b)
   if (b->d_fill)
     (void)hipFree(b->d_fill);
   for (Group& g : b->groups)
+  {
     free_lattices(g);
+    if (g.zdev)
+      (void)hipFree(g.zdev);
+  }
   delete b;
 }
 
@@ -395,7 +457,7 @@ int femtc_batch_create(femtc_batch** out, const femtc_batch_item* items,
     }
     Group& g = b->groups[f->second];
     const int cpw = cells_per_wg(it.nq);
-    BatchDesc d;
+    BatchDesc d{};
     d.tab = it.tables;
     d.x = it.x;
     d.y = it.y;
@@ -574,17 +636,24 @@ int femtc_batch_autotune(femtc_batch* b, int reps)
     // k = -1: the assembled scatter per problem (structured groups, when
     // its buffers fit); k = -2: the z-run scatter per problem (structured
     // groups, nq <= 8)
-    // (every candidate timed in two rounds, the faster round kept, so a
-    // candidate is not judged on one disturbed measurement)
+    // k = -3: one fused z-run launch over the group's problems (the same
+    // condition).  Every candidate is timed in two rounds, the faster round
+    // kept, so a candidate is not judged on one disturbed measurement.
     const bool try_asm = g.structured && ensure_lattices(g) == FEMTC_OK;
     const bool try_zrun = g.structured && g.req.gather && g.req.Q <= 8;
+    const bool try_fzrun = try_zrun && ensure_zrun(g) == FEMTC_OK;
     for (int round = 0; round < 2 && st == FEMTC_OK; ++round)
-      for (int k : {0, 1, 2, 4, 8, -1, -2})
+      for (int k : {0, 1, 2, 4, 8, -1, -2, -3})
       {
-        if ((k == -1 && !try_asm) || (k == -2 && !try_zrun))
+        if ((k == -1 && !try_asm) || (k == -2 && !try_zrun) ||
+            (k == -3 && !try_fzrun))
           continue;
         Group t = g;
-        t.per_problem = k == 0 ? 1 : k == -1 ? 2 : k == -2 ? 3 : 0;
+        t.per_problem = k == 0    ? 1
+                        : k == -1 ? 2
+                        : k == -2 ? 3
+                        : k == -3 ? 4
+                                  : 0;
         t.req.per_wg = k > 0 ? k : 1;
         st = launch_group(t, b->stream); // warm-up
         (void)hipEventRecord(e0, b->stream);
@@ -665,14 +734,14 @@ int femtc_batch_group(const femtc_batch* b, int g, int* op, int* dtype,
   if (per_wg)
   {
     const int pp = b->groups[g].per_problem;
-    *per_wg = pp == 3 ? -2 : pp == 2 ? -1 : pp ? 0 : r.per_wg;
+    *per_wg = pp == 4 ? -3 : pp == 3 ? -2 : pp == 2 ? -1 : pp ? 0 : r.per_wg;
   }
   return FEMTC_OK;
 }
 
 int femtc_batch_set_group(femtc_batch* b, int g, int per_wg)
 {
-  if (!b || g < 0 || g >= (int)b->groups.size() || per_wg < -2 ||
+  if (!b || g < 0 || g >= (int)b->groups.size() || per_wg < -3 ||
       per_wg > 64)
     return fail(FEMTC_ERR_INVALID_ARGUMENT, "no such group or bad per_wg");
   Group& grp = b->groups[g];
@@ -686,7 +755,17 @@ int femtc_batch_set_group(femtc_batch* b, int g, int per_wg)
     return fail(FEMTC_ERR_INVALID_ARGUMENT,
                 "z-run scatter: every problem of the group needs a verified "
                 "structured grid, and nq <= 8");
-  grp.per_problem = per_wg == 0 ? 1 : per_wg == -1 ? 2 : per_wg == -2 ? 3 : 0;
+  if (per_wg == -3)
+  {
+    const int st = ensure_zrun(grp);
+    if (st != FEMTC_OK)
+      return st;
+  }
+  grp.per_problem = per_wg == 0    ? 1
+                    : per_wg == -1 ? 2
+                    : per_wg == -2 ? 3
+                    : per_wg == -3 ? 4
+                                   : 0;
   grp.req.per_wg = per_wg > 0 ? per_wg : 1;
   if (grp.per_problem != 2)
     free_lattices(grp);

@@ -120,6 +120,7 @@ struct BatchDesc
   const int32_t* dofmap; // NULL for cell-local problems
   long long num_cells;
   long long blk_begin; // first workgroup-sized block of this problem
+  int grid[3];         // the z-run launch: the problem's structured grid
 };
 
 // Problems per batched launch (their first blocks are searched in LDS).
@@ -136,6 +137,11 @@ struct BatchReq
   int nitems = 0;
   long long nblk_total = 0;
   int per_wg = 1;
+  // 1: the z-run scatter (items with their grids; blocks are z-segments,
+  // blk_begin the problem's first segment); with zrun_shape set the
+  // launcher stores (cells per segment, waves) there and launches nothing
+  int zrun = 0;
+  int* zrun_shape = nullptr;
   hipStream_t stream = nullptr;
 };
 

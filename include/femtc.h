@@ -411,15 +411,17 @@ int femtc_batch_graph_launch(femtc_batch* batch);
 /* Time, for every group, one launch per problem, the fused launch with
  * 1, 2, 4 and 8 blocks per workgroup and, when every item of the group has
  * a verified grid, the assembled and (nq <= 8) z-run scatters per problem
- * (reps launches each, in two rounds, after four warm-up runs of the whole
- * batch) and keep the fastest; synchronous; outputs are left undefined. */
+ * and one fused z-run launch over the group (reps launches each, in two
+ * rounds, after four warm-up runs of the whole batch) and keep the
+ * fastest; synchronous; outputs are left undefined. */
 int femtc_batch_autotune(femtc_batch* batch, int reps);
 int femtc_batch_stats(const femtc_batch* batch, int* ngroups,
                       int64_t* total_cells, int64_t* total_dofs);
 /* per_wg = 0: the group runs one launch per problem; per_wg = k > 0: one
  * fused launch, k blocks per workgroup; per_wg = -1: the assembled scatter
  * per problem (its lattice buffers are allocated then, C*(P+1)^3 values per
- * item); per_wg = -2: the z-run scatter per problem (nq <= 8); -1 and -2
+ * item); per_wg = -2: the z-run scatter per problem (nq <= 8); per_wg =
+ * -3: one fused z-run launch over the group's problems (nq <= 8); -1..-3
  * give FEMTC_ERR_INVALID_ARGUMENT unless every item has a verified grid.
  * set_group drops a captured graph. */
 int femtc_batch_set_group(femtc_batch* batch, int group, int per_wg);

@@ -136,20 +136,21 @@ def test_batch_autotune_keeps_results():
     b.graph_launch()  # a graph captured before tuning is dropped by it
     b.autotune(reps=2)
     for g in b.groups():
-        assert g["per_wg"] in (0, 1, 2, 4, 8) or (g["per_wg"] in (-1, -2) and
-                                                  g["gather"] == 1)
+        assert g["per_wg"] in (0, 1, 2, 4, 8) or (
+            g["per_wg"] in (-1, -2, -3) and g["gather"] == 1)
     b.graph_launch()
     torch.cuda.synchronize()
     _check(items, 1e-12)
 
 
-@pytest.mark.parametrize("scatter", [-1, -2])
+@pytest.mark.parametrize("scatter", [-1, -2, -3])
 @pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-12),
                                        (torch.float32, 1e-5)])
 def test_batch_structured_scatters_vs_alone_and_oracle(dtype, tol, scatter):
-    # continuous groups on the assembled (-1) or z-run (-2) scatter (every
-    # item carries its verified grid), cell-local groups as they are; eager,
-    # graph replay, and a graph replay after eager work
+    # continuous groups on the assembled (-1) or z-run (-2, per problem;
+    # -3, one fused launch per group) scatter (every item carries its
+    # verified grid), cell-local groups as they are; eager, graph replay,
+    # and a graph replay after eager work
     items = _items(dtype, seed0=800)
     b = F.Batch(items)
     for g, info in enumerate(b.groups()):
@@ -188,8 +189,10 @@ def test_batch_assembled_scatter_needs_structured_items():
         b.set_group(ops[it["op"]], -1)
     with pytest.raises(F.FemtcError, match="structured grid"):
         b.set_group(ops[it["op"]], -2)
+    with pytest.raises(F.FemtcError, match="structured grid"):
+        b.set_group(ops[it["op"]], -3)
     b.autotune(reps=1)
-    assert b.groups()[ops[it["op"]]]["per_wg"] not in (-1, -2)
+    assert b.groups()[ops[it["op"]]]["per_wg"] not in (-1, -2, -3)
     b.run()
     _check(items, 1e-12)
 
