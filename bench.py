@@ -470,14 +470,26 @@ def bench_apply_p4(torch, F, steps=30, warmup=60):
                 "gdofs": pr["ng"] / (ms * 1e-3) / 1e9, "kernel_ms": kern,
                 "roofline_frac": ab / (kern * 1e-3) / 1e9 / HBM_PEAK_GBS}}
 
+def _sumfact_madds(op, N, Q):
+    """Multiply-adds per cell of the sum-factorized chain both high-degree
+    kernels run: mass N^3Q + N^2Q^2 + NQ^3 each way plus Q^3; stiffness
+    2N^3Q + 3N^2Q^2 + 3NQ^3 each way plus 9Q^3 for S(q)."""
+    if op == "mass":
+        return 2 * (N ** 3 * Q + N ** 2 * Q ** 2 + N * Q ** 3) + Q ** 3
+    return 2 * (2 * N ** 3 * Q + 3 * N ** 2 * Q ** 2 + 3 * N * Q ** 3) + \
+        9 * Q ** 3
+
+
 def bench_high_degree(torch, F, reps=10):
-    """Cell-local mass and stiffness applies at degrees 9..25 (the
-    general-degree kernel), Q = P+1, about 2e7 cell-local dofs (at least 2048
-    cells), U, G and detJw ~ U[0, 1) from torch's seeded generator; FP64 and
-    FP32.  Reports the time per apply, cell-local GDoF/s and algorithmic GB/s
-    (U + W + geometry).  Run with --only high_degree."""
+    """Cell-local mass and stiffness applies at degrees 9..25, Q = P+1, about
+    2e7 cell-local dofs (at least 2048 cells), U, G and detJw ~ U[0, 1) from
+    torch's seeded generator; FP64 and FP32.  Each case runs with the
+    default kernel choice (the line kernels for degrees 9..19, the general
+    kernel above) and with the general kernel forced.  Reports the time per
+    apply, cell-local GDoF/s, algorithmic GB/s (U + W + geometry) and the
+    multiply-add rate in TFLOP/s, for both.  Run with --only high_degree."""
     out = {}
-    for P in (9, 12, 16, 20, 25):
+    for P in (9, 12, 16, 19, 20, 25):
         N = Q = P + 1
         C = max(2048, int(2.0e7 / N ** 3))
         gen = torch.Generator(device="cuda").manual_seed(P)
@@ -497,22 +509,34 @@ def bench_high_degree(torch, F, reps=10):
                         F.mass_apply(t, u, w, d, C, P, Q)
                     else:
                         F.stiffness_apply(t, u, w, g, C, P, Q)
-                for _ in range(3):
-                    run()
-                e0 = torch.cuda.Event(enable_timing=True)
-                e1 = torch.cuda.Event(enable_timing=True)
-                e0.record()
-                for _ in range(reps):
-                    run()
-                e1.record()
-                torch.cuda.synchronize()
-                ms = e0.elapsed_time(e1) / reps
+
+                def time_ms():
+                    for _ in range(3):
+                        run()
+                    e0 = torch.cuda.Event(enable_timing=True)
+                    e1 = torch.cuda.Event(enable_timing=True)
+                    e0.record()
+                    for _ in range(reps):
+                        run()
+                    e1.record()
+                    torch.cuda.synchronize()
+                    return e0.elapsed_time(e1) / reps
+                ms = time_ms()
+                with F.high_degree_kernel(F.HIGH_DEGREE_GENERAL):
+                    ms_gen = time_ms()
                 es = 8 if dt == torch.float64 else 4
                 geo = Q ** 3 * (1 if op == "mass" else 6)
+                flops = 2 * _sumfact_madds(op, N, Q) * C
                 out[f"P{P}_{op}_{name}"] = {
-                    "cells": C, "us": ms * 1e3,
+                    "cells": C,
+                    "kernel": "line" if 10 <= N <= 20 else "general",
+                    "us": ms * 1e3,
                     "gdofs": C * N ** 3 / (ms * 1e-3) / 1e9,
-                    "GBs": es * C * (2 * N ** 3 + geo) / (ms * 1e-3) / 1e9}
+                    "GBs": es * C * (2 * N ** 3 + geo) / (ms * 1e-3) / 1e9,
+                    "TFLOPs": flops / (ms * 1e-3) / 1e12,
+                    "general_us": ms_gen * 1e3,
+                    "general_gdofs": C * N ** 3 / (ms_gen * 1e-3) / 1e9,
+                    "speedup_vs_general": ms_gen / ms}
             del t, u, g, d, w
         del U, G, D
         torch.cuda.empty_cache()

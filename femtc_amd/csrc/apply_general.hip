@@ -1,7 +1,10 @@
 // General-degree cell operators: N = 10..26 dofs per direction (degrees
 // 9..25, the top of the reference's dispatch ladder, src/mass.hpp:52-138 and
 // src/stiffness.hpp:102-188), Q in {N, N+1, N+2}, FP64 and FP32, cell-local
-// or gathered through a dofmap with an atomic scatter.
+// or gathered through a dofmap with an atomic scatter.  Mass, stiffness and
+// K+M in correct mode at N = 10..20 go to the line kernels (apply_line.hpp)
+// unless femtc_set_high_degree_kernel asks for this one; the reference
+// modes, the affine-geometry operators and N = 21..26 always run here.
 //
 // The fused kernels hold a whole cell per wave in registers and LDS, which
 // stops at N = 9.  Here one workgroup (256 to 768 threads) takes one cell at a time
@@ -28,6 +31,7 @@
 // (src/stiffness.hpp:27,68-97) and the literal mass is the correct chain
 // with the aliased table (src/mass.hpp:20-21,36-47).
 #include <algorithm>
+#include <atomic>
 
 #include "common.hpp"
 
@@ -402,6 +406,47 @@ int gen_op(const ApplyReq& r)
   return FEMTC_ERR_INVALID_ARGUMENT;
 }
 
+std::atomic<int> g_high_degree_mode{FEMTC_HIGH_DEGREE_AUTO};
+
+bool line_disabled()
+{
+  return g_high_degree_mode.load(std::memory_order_relaxed) ==
+         FEMTC_HIGH_DEGREE_GENERAL;
+}
+
+// The line kernels (apply_line.hpp) for N = 10..20 in correct mode: mass,
+// stiffness and K+M with per-point geometry, cell-local or gathered.
+// Returns FEMTC_ERR_INVALID_ARGUMENT when no line kernel covers the request
+// (the general kernel then runs).
+int line_launch(const ApplyReq& r)
+{
+  if (r.reference_mode || (r.op != OP_MASS && r.op != OP_STIFF &&
+                           r.op != OP_KPM))
+    return FEMTC_ERR_INVALID_ARGUMENT;
+  if (line_disabled())
+    return FEMTC_ERR_INVALID_ARGUMENT;
+  int dev = 0, max_lds = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock,
+                            dev) != hipSuccess)
+    return FEMTC_ERR_HIP;
+  switch (r.N)
+  {
+  case 10: return launch_line_n10(r, max_lds);
+  case 11: return launch_line_n11(r, max_lds);
+  case 12: return launch_line_n12(r, max_lds);
+  case 13: return launch_line_n13(r, max_lds);
+  case 14: return launch_line_n14(r, max_lds);
+  case 15: return launch_line_n15(r, max_lds);
+  case 16: return launch_line_n16(r, max_lds);
+  case 17: return launch_line_n17(r, max_lds);
+  case 18: return launch_line_n18(r, max_lds);
+  case 19: return launch_line_n19(r, max_lds);
+  case 20: return launch_line_n20(r, max_lds);
+  }
+  return FEMTC_ERR_INVALID_ARGUMENT;
+}
+
 } // namespace
 
 int launch_apply_general(const ApplyReq& r)
@@ -415,6 +460,9 @@ int launch_apply_general(const ApplyReq& r)
     return FEMTC_ERR_INVALID_ARGUMENT;
   if (r.num_cells <= 0)
     return FEMTC_OK;
+  const int st = line_launch(r);
+  if (st != FEMTC_ERR_INVALID_ARGUMENT)
+    return st;
   if (r.dtype == FEMTC_F64)
     return gen_op<double>(r);
   if (r.dtype == FEMTC_F32)
@@ -423,3 +471,15 @@ int launch_apply_general(const ApplyReq& r)
 }
 
 } // namespace femtc
+
+extern "C" int femtc_set_high_degree_kernel(int mode, int* previous)
+{
+  if (mode != FEMTC_HIGH_DEGREE_AUTO && mode != FEMTC_HIGH_DEGREE_GENERAL)
+    return femtc::fail(FEMTC_ERR_INVALID_ARGUMENT,
+                       "high-degree kernel mode must be FEMTC_HIGH_DEGREE_AUTO "
+                       "or FEMTC_HIGH_DEGREE_GENERAL");
+  const int prev = femtc::g_high_degree_mode.exchange(mode);
+  if (previous)
+    *previous = prev;
+  return FEMTC_OK;
+}

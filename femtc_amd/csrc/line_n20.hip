@@ -1,0 +1,11 @@
+// Line-kernel instances for N = 20 dofs per direction (degree 19),
+// Q in {N, N+1, N+2}, mass / stiffness / K+M, FP64 and FP32.
+#include "apply_line.hpp"
+
+namespace femtc
+{
+int launch_line_n20(const ApplyReq& r, int max_lds)
+{
+  return line_dispatch<20>(r, max_lds);
+}
+} // namespace femtc

@@ -11,6 +11,8 @@
 """
 from __future__ import annotations
 
+import contextlib
+
 import numpy as np
 
 from . import _lib
@@ -127,6 +129,29 @@ def kpm_apply(tables, U, W, G, detJw, num_cells: int, degree: int, nq: int,
     _lib.check(_lib.lib().femtc_kpm_apply(
         _dtype_code(U), degree, nq, _ptr(tables), _ptr(U), _ptr(W), _ptr(G),
         _ptr(detJw), num_cells, _stream(stream)), "kpm_apply")
+
+
+def set_high_degree_kernel(mode: int) -> int:
+    """Kernel choice for degrees 9..25 (femtc_set_high_degree_kernel):
+    HIGH_DEGREE_AUTO runs the line kernels where they apply (degree 9..19,
+    correct mode, mass / stiffness / K+M with per-point geometry), else the
+    general kernel; HIGH_DEGREE_GENERAL the general kernel always.
+    Process-wide; returns the previous mode."""
+    prev = _lib.C.c_int32(0)
+    _lib.check(_lib.lib().femtc_set_high_degree_kernel(mode,
+                                                       _lib.C.byref(prev)),
+               "set_high_degree_kernel")
+    return prev.value
+
+
+@contextlib.contextmanager
+def high_degree_kernel(mode: int):
+    """set_high_degree_kernel(mode) for the duration of a with-block."""
+    prev = set_high_degree_kernel(mode)
+    try:
+        yield
+    finally:
+        set_high_degree_kernel(prev)
 
 
 def fill_zero(y, stream=None) -> None:

@@ -191,6 +191,18 @@ int femtc_global_apply_add(int op, int dtype, int degree, int nq,
 /* y[0..n) = 0 with a kernel (usable under graph capture). */
 int femtc_fill_zero(void* y, int64_t n, int dtype, void* stream);
 
+/* Kernel choice for degrees above FEMTC_MAX_FUSED_DEGREE (no counterpart
+ * in the reference, whose ladder compiles one CPU loop per degree).
+ * FEMTC_HIGH_DEGREE_AUTO: the line kernels where they apply (N = degree+1 in
+ * 10..20, correct mode, mass / stiffness / K+M with per-point geometry,
+ * within the device's LDS per workgroup), else the general kernel;
+ * FEMTC_HIGH_DEGREE_GENERAL: the general kernel always.  Process-wide;
+ * the previous setting goes to *previous (may be NULL).  Another mode:
+ * FEMTC_ERR_INVALID_ARGUMENT, the setting unchanged. */
+#define FEMTC_HIGH_DEGREE_AUTO 0
+#define FEMTC_HIGH_DEGREE_GENERAL 1
+int femtc_set_high_degree_kernel(int mode, int* previous);
+
 /* Streaming probe (diagnostic; no counterpart in the reference): the
  * practical HBM bandwidth of a read-dominated stream like the operator
  * apply.  dst[i] = sum_r src[r*n + i] over nread (1..8) arrays of n (even)

@@ -126,6 +126,22 @@ def test_argument_errors_without_device_work():
     assert "dphi" in _lib.last_error()
 
 
+def test_high_degree_kernel_switch_without_device_work():
+    # process-wide choice between the line kernels and the general kernel
+    assert F.set_high_degree_kernel(F.HIGH_DEGREE_GENERAL) == F.HIGH_DEGREE_AUTO
+    with F.high_degree_kernel(F.HIGH_DEGREE_AUTO):
+        assert F.set_high_degree_kernel(F.HIGH_DEGREE_AUTO) == \
+            F.HIGH_DEGREE_AUTO
+    # restored on leaving the block
+    assert F.set_high_degree_kernel(F.HIGH_DEGREE_AUTO) == \
+        F.HIGH_DEGREE_GENERAL
+    prev = _lib.C.c_int32(-7)
+    assert F.lib().femtc_set_high_degree_kernel(2, _lib.C.byref(prev)) == \
+        _lib.ERR_INVALID_ARGUMENT
+    assert prev.value == -7 and "FEMTC_HIGH_DEGREE_AUTO" in _lib.last_error()
+    assert F.lib().femtc_set_high_degree_kernel(F.HIGH_DEGREE_AUTO, None) == 0
+
+
 def test_solver_create_rejects_bad_problems_without_touching_the_device():
     # argument checks run before any HIP call
     import ctypes as C_

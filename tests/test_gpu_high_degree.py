@@ -1,8 +1,10 @@
-"""GPU parity of the general-degree kernels (degrees 9..25, the top of the
+"""GPU parity of the high-degree kernels (degrees 9..25, the top of the
 reference's dispatch ladder) through the C ABI against the CPU oracle:
 cell-local mass, stiffness and K+M, the reference's literal mass and
 stiffness, the dofmap gather with the atomic scatter, the affine-geometry
-variants and the reference-signature host shim.
+variants and the reference-signature host shim.  The operator tests run
+twice, with the line kernels where they apply (degrees 9..19) and with the
+general kernel forced (femtc_set_high_degree_kernel).
 
 Bars: FP64 relative l2 <= 1e-12 against the oracle (whose literal operators
 match the compiled reference bit for bit at these degrees,
@@ -21,6 +23,13 @@ pytestmark = pytest.mark.gpu
 
 TOL64 = 1e-12
 TOL32 = 1e-5
+
+
+@pytest.fixture(params=[F.HIGH_DEGREE_AUTO, F.HIGH_DEGREE_GENERAL],
+                ids=["line", "general"])
+def hd_kernel(request):
+    with F.high_degree_kernel(request.param):
+        yield request.param
 
 
 def dev(a, dtype=torch.float64):
@@ -65,7 +74,7 @@ CELL_SHAPES = [(P, Q) for P in (9, 10, 13, 15, 17, 21, 25)
 
 @pytest.mark.parametrize("P,Q", CELL_SHAPES)
 @pytest.mark.parametrize("op", ["mass", "stiffness", "kpm"])
-def test_cell_operator_f64_vs_oracle(op, P, Q):
+def test_cell_operator_f64_vs_oracle(op, P, Q, hd_kernel):
     ncell = 3
     print(f"general-degree cell {op} P={P} Q={Q}", flush=True)
     N, tab, U, G, Dw = _case(P, Q, ncell, 40 * P + Q, table="random")
@@ -75,9 +84,9 @@ def test_cell_operator_f64_vs_oracle(op, P, Q):
     assert rel(host(W), ref) <= TOL64, (op, P, Q)
 
 
-@pytest.mark.parametrize("P", [9, 17, 25])
+@pytest.mark.parametrize("P", [9, 16, 17, 19, 25])
 @pytest.mark.parametrize("op", ["mass", "stiffness", "kpm"])
-def test_cell_operator_f32_vs_oracle(op, P):
+def test_cell_operator_f32_vs_oracle(op, P, hd_kernel):
     ncell, Q = 4, P + 1
     N, tab, U, G, Dw = _case(P, Q, ncell, 3 * P)
     ref = O.cell_operator(op, N, Q, tab, U, G, Dw, ncell)
@@ -86,6 +95,54 @@ def test_cell_operator_f32_vs_oracle(op, P):
     err = rel(host(W), ref)
     print(f"FP32 general-degree {op} P={P}: {err:.2e} vs the FP64 oracle")
     assert err <= TOL32, (op, P)
+
+
+LINE_SHAPES = [(P, Q) for P in range(9, 20) for Q in (P + 1, P + 2, P + 3)]
+
+
+@pytest.mark.parametrize("P,Q", LINE_SHAPES)
+def test_line_kernels_every_shape_vs_oracle(P, Q):
+    # every (degree, nq) the line kernels instantiate, all three operators,
+    # FP64 against the oracle on random tables and geometry; 3 cells, so a
+    # workgroup of two cells also runs with its second cell past the end
+    ncell = 3
+    print(f"line kernels P={P} Q={Q}", flush=True)
+    N, tab, U, G, Dw = _case(P, Q, ncell, 11 * P + Q, table="random")
+    for op in ("mass", "stiffness", "kpm"):
+        ref = O.cell_operator(op, N, Q, tab, U, G, Dw, ncell)
+        W = torch.full((ncell * N ** 3,), -5.0, device="cuda",
+                       dtype=torch.float64)
+        _apply(op, torch.float64, tab, U, W, G, Dw, ncell, P, Q)
+        assert rel(host(W), ref) <= TOL64, (op, P, Q)
+
+
+@pytest.mark.parametrize("P,Q,ncell", [(9, 10, 16500), (16, 17, 701),
+                                        (15, 16, 33001)])
+@pytest.mark.parametrize("op", ["mass", "stiffness"])
+def test_line_kernel_matches_general_kernel_at_size(op, P, Q, ncell):
+    # many cells (16500 > the 16384-workgroup grid, and 33001 cells in
+    # workgroups of two, so workgroups loop over cells; odd counts leave a
+    # workgroup's second cell past the end): the line kernel against the
+    # general kernel on the GPU, which differ only in the order of the sums
+    N, tab, U, G, Dw = _case(P, Q, 1, 5 * P)
+    U = O.uniform(91, ncell * N ** 3)
+    dU, dtab = dev(U), dev(tab)
+    if op == "mass":
+        geo = torch.rand(ncell * Q ** 3, device="cuda", dtype=torch.float64)
+        geo += 0.5
+    else:
+        geo = torch.rand(ncell * Q ** 3 * 6, device="cuda",
+                         dtype=torch.float64)
+    out = {}
+    for mode in (F.HIGH_DEGREE_AUTO, F.HIGH_DEGREE_GENERAL):
+        W = torch.empty(ncell * N ** 3, device="cuda", dtype=torch.float64)
+        with F.high_degree_kernel(mode):
+            if op == "mass":
+                F.mass_apply(dtab, dU, W, geo, ncell, P, Q)
+            else:
+                F.stiffness_apply(dtab, dU, W, geo, ncell, P, Q)
+        out[mode] = host(W)
+    assert rel(out[F.HIGH_DEGREE_AUTO], out[F.HIGH_DEGREE_GENERAL]) <= TOL64
 
 
 @pytest.mark.parametrize("P", [9, 12, 20, 25])
@@ -138,7 +195,7 @@ def _box(nx, ny, nz, P, Q, seed):
 @pytest.mark.parametrize("nx,ny,nz,P,Q", [(2, 2, 2, 9, 10), (2, 1, 2, 11, 13),
                                           (1, 2, 1, 16, 17), (1, 1, 2, 24, 25)])
 @pytest.mark.parametrize("op", [F.OP_MASS, F.OP_STIFFNESS, F.OP_KPM])
-def test_global_operator_vs_oracle(op, nx, ny, nz, P, Q):
+def test_global_operator_vs_oracle(op, nx, ny, nz, P, Q, hd_kernel):
     C, J, G, Dw, dm, ng = _box(nx, ny, nz, P, Q, 60 + P)
     tab = O.basis_tables(P, Q)
     x = O.uniform(61, ng)
