@@ -44,7 +44,7 @@ constexpr int LINE_NMAX = 20;
 #define LINE_KPM64_LIVE 66
 #endif
 #ifndef LINE_PREFETCH
-#define LINE_PREFETCH 1
+#define LINE_PREFETCH 2
 #endif
 #ifndef LINE_PAIR
 #define LINE_PAIR 2
