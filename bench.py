@@ -484,12 +484,12 @@ def bench_high_degree(torch, F, reps=10):
     """Cell-local mass and stiffness applies at degrees 9..25, Q = P+1, about
     2e7 cell-local dofs (at least 2048 cells), U, G and detJw ~ U[0, 1) from
     torch's seeded generator; FP64 and FP32.  Each case runs with the
-    default kernel choice (the line kernels for degrees 9..19, the general
-    kernel above) and with the general kernel forced.  Reports the time per
+    default kernel choice (the line kernels where one exists for the shape,
+    femtc_apply_kernel) and with the general kernel forced.  Reports the time per
     apply, cell-local GDoF/s, algorithmic GB/s (U + W + geometry) and the
     multiply-add rate in TFLOP/s, for both.  Run with --only high_degree."""
     out = {}
-    for P in (9, 12, 16, 19, 20, 25):
+    for P in (9, 12, 16, 19, 20, 21, 25):
         N = Q = P + 1
         C = max(2048, int(2.0e7 / N ** 3))
         gen = torch.Generator(device="cuda").manual_seed(P)
@@ -527,9 +527,11 @@ def bench_high_degree(torch, F, reps=10):
                 es = 8 if dt == torch.float64 else 4
                 geo = Q ** 3 * (1 if op == "mass" else 6)
                 flops = 2 * _sumfact_madds(op, N, Q) * C
+                kop = F.OP_MASS if op == "mass" else F.OP_STIFFNESS
+                kdt = F.F64 if dt == torch.float64 else F.F32
                 out[f"P{P}_{op}_{name}"] = {
                     "cells": C,
-                    "kernel": "line" if 10 <= N <= 20 else "general",
+                    "kernel": F.apply_kernel(kop, kdt, P, Q),
                     "us": ms * 1e3,
                     "gdofs": C * N ** 3 / (ms * 1e-3) / 1e9,
                     "GBs": es * C * (2 * N ** 3 + geo) / (ms * 1e-3) / 1e9,

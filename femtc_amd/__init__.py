@@ -12,8 +12,9 @@ from .linalg import Order, string2order
 from .mesh import (basis_tables, basis_tables_weights, dofmap_structured,
                    even_odd_tables, gauss_rule, geometry_affine,
                    geometry_cell, gll_points)
-from .operators import (fill_zero, global_apply, global_apply_affine,
-                        high_degree_kernel, kpm_apply, mass_apply,
+from .operators import (apply_kernel, fill_zero, global_apply,
+                        global_apply_affine, high_degree_kernel, kpm_apply,
+                        mass_apply,
                         mass_operator, set_high_degree_kernel,
                         stiffness_apply, stiffness_operator)
 from .mfma import global_apply_mfma, mfma_tables, mfma_tables_bytes
@@ -28,7 +29,7 @@ __all__ = [
     "BF16", "BF16_ROUNDED", "F32", "F64", "MODE_CORRECT", "MODE_REFERENCE", "OP_KPM", "OP_MASS",
     "OP_STIFFNESS", "SCATTER_ASSEMBLED", "SCATTER_AUTO", "SCATTER_DIRECT",
     "SCATTER_ZRUN", "HIGH_DEGREE_AUTO", "HIGH_DEGREE_GENERAL",
-    "high_degree_kernel", "set_high_degree_kernel",
+    "high_degree_kernel", "set_high_degree_kernel", "apply_kernel",
     "FemtcError", "lib", "Order", "string2order", "fill_zero",
     "basis_tables", "dofmap_structured", "gauss_rule", "geometry_affine",
     "gll_points", "global_apply", "kpm_apply", "mass_apply", "mass_operator",

@@ -64,6 +64,8 @@ SIGNATURES = {
     "femtc_stream_probe": (_i32, [_i32, _i32, _vp, _vp, _i64, _vp, _i32,
                                   _i32, _vp]),
     "femtc_set_high_degree_kernel": (_i32, [_i32, C.POINTER(_i32)]),
+    "femtc_apply_kernel": (_i32, [_i32, _i32, _i32, _i32, _i32,
+                                  C.POINTER(_i32)]),
     "femtc_mass_operator": (_i32, [_i32, _vp, _sz, _vp, _vp, _vp, _sz, _i32,
                                    _i32, _i32, _i32, _i32]),
     "femtc_stiffness_operator": (_i32, [_i32, _vp, _sz, _vp, _vp, _vp, _i32,
@@ -142,6 +144,7 @@ class Problem(C.Structure):
 SCATTER_AUTO, SCATTER_DIRECT, SCATTER_ASSEMBLED, SCATTER_ZRUN = 0, 1, 2, 3
 
 HIGH_DEGREE_AUTO, HIGH_DEGREE_GENERAL = 0, 1
+KERNEL_FUSED, KERNEL_GENERAL, KERNEL_LINE = 0, 1, 2
 
 
 class BatchItem(C.Structure):

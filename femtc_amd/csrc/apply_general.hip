@@ -2,9 +2,11 @@
 // 9..25, the top of the reference's dispatch ladder, src/mass.hpp:52-138 and
 // src/stiffness.hpp:102-188), Q in {N, N+1, N+2}, FP64 and FP32, cell-local
 // or gathered through a dofmap with an atomic scatter.  Mass, stiffness and
-// K+M in correct mode at N = 10..20 go to the line kernels (apply_line.hpp)
-// unless femtc_set_high_degree_kernel asks for this one; the reference
-// modes, the affine-geometry operators and N = 21..26 always run here.
+// K+M in correct mode go to the line kernels (apply_line.hpp) where one
+// exists for (dtype, op, N, Q) -- N = 10..22 in FP64 stiffness, up to 26
+// for the rest (line_fits) -- unless femtc_set_high_degree_kernel asks for
+// this one; the reference modes, the affine-geometry operators and the
+// shapes without a line kernel run here.
 //
 // The fused kernels hold a whole cell per wave in registers and LDS, which
 // stops at N = 9.  Here one workgroup (256 to 768 threads) takes one cell at a time
@@ -33,6 +35,7 @@
 #include <algorithm>
 #include <atomic>
 
+#include "apply_line.hpp"
 #include "common.hpp"
 
 namespace femtc
@@ -414,7 +417,7 @@ bool line_disabled()
          FEMTC_HIGH_DEGREE_GENERAL;
 }
 
-// The line kernels (apply_line.hpp) for N = 10..20 in correct mode: mass,
+// The line kernels (apply_line.hpp) for N = 10..26 in correct mode: mass,
 // stiffness and K+M with per-point geometry, cell-local or gathered.
 // Returns FEMTC_ERR_INVALID_ARGUMENT when no line kernel covers the request
 // (the general kernel then runs).
@@ -443,6 +446,12 @@ int line_launch(const ApplyReq& r)
   case 18: return launch_line_n18(r, max_lds);
   case 19: return launch_line_n19(r, max_lds);
   case 20: return launch_line_n20(r, max_lds);
+  case 21: return launch_line_n21(r, max_lds);
+  case 22: return launch_line_n22(r, max_lds);
+  case 23: return launch_line_n23(r, max_lds);
+  case 24: return launch_line_n24(r, max_lds);
+  case 25: return launch_line_n25(r, max_lds);
+  case 26: return launch_line_n26(r, max_lds);
   }
   return FEMTC_ERR_INVALID_ARGUMENT;
 }
@@ -471,6 +480,72 @@ int launch_apply_general(const ApplyReq& r)
 }
 
 } // namespace femtc
+
+namespace femtc
+{
+namespace
+{
+// line_fits for a runtime (dtype, op, N, Q), N = LINE_NMIN..LINE_NMAX,
+// Q - N in 0..2 (constexpr evaluation only; no kernel is instantiated)
+template <int N>
+bool line_fits_n(int dtype, int op, int dq)
+{
+  if constexpr (N > LINE_NMAX)
+    return false;
+  else
+  {
+    auto pick = [&](auto t) {
+      using T = decltype(t);
+      auto byop = [&](auto qc) {
+        constexpr int Q = N + decltype(qc)::value;
+        return op == OP_MASS    ? line_fits<T, OP_MASS, N, Q>()
+               : op == OP_STIFF ? line_fits<T, OP_STIFF, N, Q>()
+                                : line_fits<T, OP_KPM, N, Q>();
+      };
+      return dq == 0   ? byop(std::integral_constant<int, 0>{})
+             : dq == 1 ? byop(std::integral_constant<int, 1>{})
+                       : byop(std::integral_constant<int, 2>{});
+    };
+    return dtype == FEMTC_F64 ? pick(double{}) : pick(float{});
+  }
+}
+
+template <int N>
+bool line_fits_rt(int dtype, int op, int n, int dq)
+{
+  if constexpr (N > LINE_NMAX)
+    return false;
+  else
+    return n == N ? line_fits_n<N>(dtype, op, dq)
+                  : line_fits_rt<N + 1>(dtype, op, n, dq);
+}
+} // namespace
+} // namespace femtc
+
+extern "C" int femtc_apply_kernel(int op, int dtype, int degree, int nq,
+                                  int mode, int* kernel)
+{
+  using namespace femtc;
+  if (!kernel)
+    return fail(FEMTC_ERR_INVALID_ARGUMENT, "kernel must not be null");
+  int st = check_shape(degree, nq, mode);
+  if (st != FEMTC_OK)
+    return st;
+  if ((st = check_dtype(dtype)) != FEMTC_OK)
+    return st;
+  if (op != FEMTC_OP_MASS && op != FEMTC_OP_STIFFNESS && op != FEMTC_OP_KPM)
+    return fail(FEMTC_ERR_INVALID_ARGUMENT, "op must be FEMTC_OP_*");
+  const int N = degree + 1;
+  if (degree <= FEMTC_MAX_FUSED_DEGREE)
+    *kernel = FEMTC_KERNEL_FUSED;
+  else if (mode == FEMTC_MODE_CORRECT && !line_disabled() &&
+           N >= LINE_NMIN &&
+           line_fits_rt<LINE_NMIN>(dtype, op, N, nq - N))
+    *kernel = FEMTC_KERNEL_LINE;
+  else
+    *kernel = FEMTC_KERNEL_GENERAL;
+  return FEMTC_OK;
+}
 
 extern "C" int femtc_set_high_degree_kernel(int mode, int* previous)
 {

@@ -1,7 +1,8 @@
 // Line kernels for the high degrees (the top of the reference's dispatch
 // ladder, src/mass.hpp:52-138 and src/stiffness.hpp:102-188): cell-local or
 // gathered mass, stiffness and K+M in correct mode, N = LINE_NMIN..LINE_NMAX
-// dofs per direction, Q in {N, N+1, N+2}, FP64 and FP32.
+// dofs per direction, Q in {N, N+1, N+2}, FP64 and FP32, for the shapes
+// whose registers and LDS fit (line_fits).
 //
 // One workgroup takes one cell at a time.  Every contraction is done by
 // threads that own a whole line of the tensor along the contracted axis,
@@ -36,7 +37,7 @@ namespace femtc
 {
 
 constexpr int LINE_NMIN = 10;
-constexpr int LINE_NMAX = 20;
+constexpr int LINE_NMAX = 26;
 #ifndef LINE_STIFF64_LIVE
 #define LINE_STIFF64_LIVE 73
 #endif
@@ -76,11 +77,20 @@ constexpr int line_want()
                                                                  : 3;
 }
 
-// LDS of one cell: two slices.
+// LDS slices per cell: two alternating slices (one barrier per transpose)
+// when they fit the 160 KB of a CU, else one (two barriers per transpose).
+template <typename T, int N, int Q>
+constexpr int line_nbuf()
+{
+  return 2 * (size_t)LineShape<N, Q>::SLICE * sizeof(T) <= 160 * 1024 ? 2
+                                                                      : 1;
+}
+
+// LDS of one cell.
 template <typename T, int N, int Q>
 constexpr size_t line_cell_lds()
 {
-  return 2 * (size_t)LineShape<N, Q>::SLICE * sizeof(T);
+  return line_nbuf<T, N, Q>() * (size_t)LineShape<N, Q>::SLICE * sizeof(T);
 }
 
 // Cells per workgroup: 2 when one cell takes 5 to 7 waves, its registers
@@ -88,12 +98,32 @@ constexpr size_t line_cell_lds()
 // (measured: a second workgroup of 5 waves does not become resident beside
 // the first, so one workgroup of two cells doubles the waves per CU), else
 // 1.
+// Registers a cell's values need: the values live at once (mass: a line of
+// Q and one of N; stiffness: three of Q and one of N; K+M: four of Q and one
+// of N) plus about 24 of addresses and temporaries (the compiler's counts
+// over N <= 20).
+template <typename T, int OP, int N, int Q>
+constexpr int line_vgprs()
+{
+  constexpr int live = OP == OP_MASS ? Q + N : OP == OP_STIFF ? 3 * Q + N
+                                                              : 4 * Q + N;
+  return live * (int)(sizeof(T) / 4) + 24;
+}
+
+// VGPRs per wave with w waves per SIMD (arch VGPRs, 8-register granules).
+constexpr int line_budget(int w)
+{
+  return w <= 2 ? 256 : (512 / w) / 8 * 8;
+}
+
 template <typename T, int OP, int N, int Q>
 constexpr int line_cpb()
 {
   constexpr int w1 = LineShape<N, Q>::TH / 64;
+  constexpr int w2 = (2 * w1 + 3) / 4; // waves per SIMD for two cells
   return (w1 >= 5 && w1 <= 7 && line_want<T, OP, N, Q>() == 3 &&
-          2 * line_cell_lds<T, N, Q>() <= 160 * 1024)
+          2 * line_cell_lds<T, N, Q>() <= 160 * 1024 &&
+          line_vgprs<T, OP, N, Q>() <= line_budget(w2 > 3 ? w2 : 3))
              ? LINE_PAIR
              : 1;
 }
@@ -118,6 +148,17 @@ constexpr int line_waves()
   constexpr int wg = (line_threads<T, OP, N, Q>() / 64 + 3) / 4;
   constexpr int want = line_want<T, OP, N, Q>();
   return want > wg ? want : wg;
+}
+
+// Whether a line kernel exists for (T, OP, N, Q): one cell's LDS fits a CU
+// and line_vgprs fits the registers of the waves per SIMD the workgroup
+// needs.  Other shapes run on the general kernel.
+template <typename T, int OP, int N, int Q>
+constexpr bool line_fits()
+{
+  return line_vgprs<T, OP, N, Q>() <= line_budget(line_waves<T, OP, N, Q>()) &&
+         line_cell_lds<T, N, Q>() <= 160 * 1024 &&
+         line_threads<T, OP, N, Q>() <= 1024;
 }
 
 
@@ -327,6 +368,7 @@ __global__ void __launch_bounds__((line_threads<T, OP, N, Q>()))
   // CPB cells per workgroup, each on its own S::TH threads (whole waves)
   // and its own two LDS slices
   constexpr int CPB = line_cpb<T, OP, N, Q>();
+  constexpr int NB = line_nbuf<T, N, Q>();
   extern __shared__ __align__(16) unsigned char line_smem[];
   const T* const phi = tab; // [dof][q]
   const T* const dph = tab + N * Q;
@@ -347,8 +389,8 @@ __global__ void __launch_bounds__((line_threads<T, OP, N, Q>()))
     const int t = tt - h * S::TH;
     const bool live = c0 + h < C; // a second cell past the last one
     const int64_t cell = live ? c0 + h : C - 1;
-    T* const slice0 = (T*)line_smem + h * 2 * S::SLICE;
-    T* const slice1 = slice0 + S::SLICE;
+    T* const slice0 = (T*)line_smem + h * NB * S::SLICE;
+    T* const slice1 = slice0 + (NB - 1) * S::SLICE;
     const bool own_bc = t < NN, own_ac = t < QN, own_ab = t < QQ;
     const int tbc = own_bc ? t : NN - 1;            // (b, c)
     const int tac = own_ac ? t : QN - 1;            // (qa, c)
@@ -358,11 +400,17 @@ __global__ void __launch_bounds__((line_threads<T, OP, N, Q>()))
     const int64_t cb = cell * (int64_t)(N * NN);
     const T* const xc = dofmap ? x : x + cb;
     const int32_t* const dc = dofmap ? dofmap + cb : nullptr;
-    // Transpose k writes slice k % 2 and reads it after barrier k.  A
-    // write may come as soon as barrier k-1 has passed (the slice's last
-    // reads, transpose k-2, come before it in every thread), so each array
-    // goes to LDS as early as possible and leaves the registers.
+    // Two slices: transpose k writes slice k % 2 and reads it after
+    // barrier k.  A write may come as soon as barrier k-1 has passed (the
+    // slice's last reads, transpose k-2, come before it in every thread),
+    // so each array goes to LDS as early as possible and leaves the
+    // registers.  One slice (NB = 1, large cells): every write waits on a
+    // barrier after the previous transpose's reads (wb).
     auto slice = [&](int k) { return (k & 1) ? slice1 : slice0; };
+    auto wb = [&]() {
+      if constexpr (NB == 1)
+        __syncthreads();
+    };
     // (b,c) writes [qa][b][c]; (qa,c) reads along b
     auto w_bc = [&](int k, const T (&v)[Q]) {
       if (own_bc)
@@ -436,7 +484,75 @@ __global__ void __launch_bounds__((line_threads<T, OP, N, Q>()))
     }
 
     T W[N];
-    if constexpr (FULL)
+    if constexpr (FULL && NB == 1)
+    {
+      // the same transposes on one slice, each write after the reads of
+      // the one before
+      T B0[Q], B1[Q], B2[Q], l[N];
+      wb();
+      w_bc(0, A0);
+      __syncthreads();
+      r_ac_b(0, l);
+      line_fwd<T, N, Q>(phi, l, B0);
+      line_fwd<T, N, Q>(dph, l, B1);
+      wb();
+      w_bc(1, A1);
+      __syncthreads();
+      r_ac_b(1, l);
+      line_fwd<T, N, Q>(phi, l, B2);
+      wb();
+      w_ac_q(2, B0);
+      __syncthreads();
+      T U[MT ? Q : 1], DA[Q], DB[Q], DC[Q];
+      r_ab_c(2, l);
+      if constexpr (MT)
+        line_fwd<T, N, Q>(phi, l, U);
+      line_fwd<T, N, Q>(dph, l, DC);
+      wb();
+      w_ac_q(3, B1);
+      __syncthreads();
+      r_ab_c(3, l);
+      line_fwd<T, N, Q>(phi, l, DB);
+      wb();
+      w_ac_q(4, B2);
+      __syncthreads();
+      r_ab_c(4, l);
+      line_fwd<T, N, Q>(phi, l, DA);
+      line_pointwise<T, OP, N, Q>(G, detJw, cell, tab_, DA, DB, DC, U);
+      T D1[N], D2[N], D3[N];
+      line_bwd<T, Q, N>(dph, DC, D3);
+      if constexpr (MT)
+        line_bwd<T, Q, N, true>(phi, U, D3);
+      line_bwd<T, Q, N>(phi, DA, D1);
+      line_bwd<T, Q, N>(phi, DB, D2);
+      T E1[N], E2[N], lq[Q];
+      wb();
+      w_ab(5, D3);
+      __syncthreads();
+      r_ac_qb(5, lq);
+      line_bwd<T, Q, N>(phi, lq, E2);
+      wb();
+      w_ab(6, D1);
+      __syncthreads();
+      r_ac_qb(6, lq);
+      line_bwd<T, Q, N>(phi, lq, E1);
+      wb();
+      w_ab(7, D2);
+      __syncthreads();
+      r_ac_qb(7, lq);
+      line_bwd<T, Q, N, true>(dph, lq, E2);
+      wb();
+      w_ac_b(8, E1);
+      __syncthreads();
+      r_bc_qa(8, lq);
+      line_bwd<T, Q, N>(dph, lq, W);
+      wb();
+      w_ac_b(9, E2);
+      __syncthreads();
+      r_bc_qa(9, lq);
+      line_bwd<T, Q, N, true>(phi, lq, W);
+    }
+    else if constexpr (FULL)
     {
       // transposes 0 A0, 1 A1, 2 B0, 3 B1, 4 B2, 5 D3, 6 D1, 7 D2, 8 E1,
       // 9 E2
@@ -500,12 +616,15 @@ __global__ void __launch_bounds__((line_threads<T, OP, N, Q>()))
     }
     else
     {
-      // transposes 0 A0, 1 B0, 2 D3, 3 E2
+      // transposes 0 A0, 1 B0, 2 D3, 3 E2; each write already follows the
+      // reads of the transpose before
       T B0[Q], U[Q], D3[N], E2[N], l[N], lq[Q];
+      wb();
       w_bc(0, A0);
       __syncthreads();
       r_ac_b(0, l);
       line_fwd<T, N, Q>(phi, l, B0);
+      wb();
       w_ac_q(1, B0);
       __syncthreads();
       r_ab_c(1, l);
@@ -513,10 +632,12 @@ __global__ void __launch_bounds__((line_threads<T, OP, N, Q>()))
       T none[1];
       line_pointwise<T, OP, N, Q>(G, detJw, cell, tab_, none, none, none, U);
       line_bwd<T, Q, N>(phi, U, D3);
+      wb();
       w_ab(2, D3);
       __syncthreads();
       r_ac_qb(2, lq);
       line_bwd<T, Q, N>(phi, lq, E2);
+      wb();
       w_ac_b(3, E2);
       __syncthreads();
       r_bc_qa(3, lq);
@@ -566,14 +687,23 @@ int line_launch_op(const ApplyReq& r, int max_lds)
   return hipGetLastError() == hipSuccess ? FEMTC_OK : FEMTC_ERR_HIP;
 }
 
+template <typename T, int OP, int N, int Q>
+int line_launch_fit(const ApplyReq& r, int max_lds)
+{
+  if constexpr (line_fits<T, OP, N, Q>())
+    return line_launch_op<T, OP, N, Q>(r, max_lds);
+  else
+    return FEMTC_ERR_INVALID_ARGUMENT;
+}
+
 template <typename T, int N, int Q>
 int line_launch_q(const ApplyReq& r, int max_lds)
 {
   switch (r.op)
   {
-  case OP_MASS: return line_launch_op<T, OP_MASS, N, Q>(r, max_lds);
-  case OP_STIFF: return line_launch_op<T, OP_STIFF, N, Q>(r, max_lds);
-  case OP_KPM: return line_launch_op<T, OP_KPM, N, Q>(r, max_lds);
+  case OP_MASS: return line_launch_fit<T, OP_MASS, N, Q>(r, max_lds);
+  case OP_STIFF: return line_launch_fit<T, OP_STIFF, N, Q>(r, max_lds);
+  case OP_KPM: return line_launch_fit<T, OP_KPM, N, Q>(r, max_lds);
   }
   return FEMTC_ERR_INVALID_ARGUMENT;
 }

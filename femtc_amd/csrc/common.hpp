@@ -181,7 +181,7 @@ int launch_apply_n8(const ApplyReq&);
 int launch_apply_n9(const ApplyReq&);
 // N = 10..26 (apply_general.hip); also accepts smaller N
 int launch_apply_general(const ApplyReq&);
-// Line kernels, N = 10..20 (apply_line.hpp); max_lds = the device's LDS
+// Line kernels, N = 10..26 (apply_line.hpp); max_lds = the device's LDS
 // bytes per workgroup.  FEMTC_ERR_INVALID_ARGUMENT when the variant does not
 // exist or does not fit.
 int launch_line_n10(const ApplyReq&, int max_lds);
@@ -195,6 +195,12 @@ int launch_line_n17(const ApplyReq&, int max_lds);
 int launch_line_n18(const ApplyReq&, int max_lds);
 int launch_line_n19(const ApplyReq&, int max_lds);
 int launch_line_n20(const ApplyReq&, int max_lds);
+int launch_line_n21(const ApplyReq&, int max_lds);
+int launch_line_n22(const ApplyReq&, int max_lds);
+int launch_line_n23(const ApplyReq&, int max_lds);
+int launch_line_n24(const ApplyReq&, int max_lds);
+int launch_line_n25(const ApplyReq&, int max_lds);
+int launch_line_n26(const ApplyReq&, int max_lds);
 
 // The K+M cell pass with bfloat16 operands on the matrix cores and FP32
 // accumulation (mfma_cell.hip): affine cells of a structured ncx x ncy x ncz

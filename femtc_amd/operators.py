@@ -133,15 +133,30 @@ def kpm_apply(tables, U, W, G, detJw, num_cells: int, degree: int, nq: int,
 
 def set_high_degree_kernel(mode: int) -> int:
     """Kernel choice for degrees 9..25 (femtc_set_high_degree_kernel):
-    HIGH_DEGREE_AUTO runs the line kernels where they apply (degree 9..19,
-    correct mode, mass / stiffness / K+M with per-point geometry), else the
-    general kernel; HIGH_DEGREE_GENERAL the general kernel always.
+    HIGH_DEGREE_AUTO runs the line kernels where they apply (correct mode,
+    mass / stiffness / K+M with per-point geometry, degrees 9..25 where a
+    cell fits the registers and LDS: FP64 stiffness and K+M to degree
+    20..21), else the general kernel; HIGH_DEGREE_GENERAL the general kernel
+    always.
     Process-wide; returns the previous mode."""
     prev = _lib.C.c_int32(0)
     _lib.check(_lib.lib().femtc_set_high_degree_kernel(mode,
                                                        _lib.C.byref(prev)),
                "set_high_degree_kernel")
     return prev.value
+
+
+def apply_kernel(op: int, dtype: int, degree: int, nq: int,
+                 mode: int = 0) -> str:
+    """Which kernel family the apply entry points run for these arguments
+    under the current setting (femtc_apply_kernel): "fused", "line" or
+    "general"."""
+    k = _lib.C.c_int32(-1)
+    _lib.check(_lib.lib().femtc_apply_kernel(op, dtype, degree, nq, mode,
+                                             _lib.C.byref(k)),
+               "apply_kernel")
+    return {_lib.KERNEL_FUSED: "fused", _lib.KERNEL_GENERAL: "general",
+            _lib.KERNEL_LINE: "line"}[k.value]
 
 
 @contextlib.contextmanager

@@ -193,15 +193,28 @@ int femtc_fill_zero(void* y, int64_t n, int dtype, void* stream);
 
 /* Kernel choice for degrees above FEMTC_MAX_FUSED_DEGREE (no counterpart
  * in the reference, whose ladder compiles one CPU loop per degree).
- * FEMTC_HIGH_DEGREE_AUTO: the line kernels where they apply (N = degree+1 in
- * 10..20, correct mode, mass / stiffness / K+M with per-point geometry,
- * within the device's LDS per workgroup), else the general kernel;
+ * FEMTC_HIGH_DEGREE_AUTO: the line kernels where they apply (correct mode,
+ * mass / stiffness / K+M with per-point geometry, N = degree+1 in 10..26
+ * where a cell's registers and LDS fit: FP64 stiffness and K+M up to
+ * N = 21..22, the rest up to 26), else the general kernel;
  * FEMTC_HIGH_DEGREE_GENERAL: the general kernel always.  Process-wide;
  * the previous setting goes to *previous (may be NULL).  Another mode:
  * FEMTC_ERR_INVALID_ARGUMENT, the setting unchanged. */
 #define FEMTC_HIGH_DEGREE_AUTO 0
 #define FEMTC_HIGH_DEGREE_GENERAL 1
 int femtc_set_high_degree_kernel(int mode, int* previous);
+
+/* Which kernel family the operator apply entry points run for (op
+ * FEMTC_OP_*, dtype, degree, nq, mode FEMTC_MODE_*) under the current
+ * setting above (no counterpart in the reference): *kernel =
+ * FEMTC_KERNEL_FUSED (degrees 1..FEMTC_MAX_FUSED_DEGREE),
+ * FEMTC_KERNEL_LINE or FEMTC_KERNEL_GENERAL.  Assumes the 160 KB of LDS
+ * per workgroup of gfx950; no device call. */
+#define FEMTC_KERNEL_FUSED 0
+#define FEMTC_KERNEL_GENERAL 1
+#define FEMTC_KERNEL_LINE 2
+int femtc_apply_kernel(int op, int dtype, int degree, int nq, int mode,
+                       int* kernel);
 
 /* Streaming probe (diagnostic; no counterpart in the reference): the
  * practical HBM bandwidth of a read-dominated stream like the operator

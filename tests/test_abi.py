@@ -142,6 +142,35 @@ def test_high_degree_kernel_switch_without_device_work():
     assert F.lib().femtc_set_high_degree_kernel(F.HIGH_DEGREE_AUTO, None) == 0
 
 
+def test_apply_kernel_query_without_device_work():
+    K = F.apply_kernel
+    assert K(F.OP_STIFFNESS, F.F64, 6, 7) == "fused"
+    assert K(F.OP_STIFFNESS, F.F64, 16, 17) == "line"
+    assert K(F.OP_STIFFNESS, F.F64, 16, 17, F.MODE_REFERENCE) == "general"
+    with F.high_degree_kernel(F.HIGH_DEGREE_GENERAL):
+        assert K(F.OP_STIFFNESS, F.F64, 16, 17) == "general"
+    # FP64 stiffness: N = 21, 22 (Q = N or N+1) fit, larger cells do not
+    assert K(F.OP_STIFFNESS, F.F64, 21, 22) == "line"
+    assert K(F.OP_STIFFNESS, F.F64, 21, 23) == "general"
+    assert K(F.OP_STIFFNESS, F.F64, 25, 26) == "general"
+    # mass and FP32 reach the top of the ladder
+    assert K(F.OP_MASS, F.F64, 25, 26) == "line"
+    assert K(F.OP_STIFFNESS, F.F32, 25, 26) == "line"
+    assert K(F.OP_STIFFNESS, F.F32, 25, 28) == "general"
+    for P in range(9, 26):
+        for dq in (1, 2, 3):
+            for op in (F.OP_MASS, F.OP_STIFFNESS, F.OP_KPM):
+                for dt in (F.F64, F.F32):
+                    assert K(op, dt, P, P + dq) in ("line", "general")
+    k = _lib.C.c_int32(-1)
+    assert F.lib().femtc_apply_kernel(7, F.F64, 16, 17, 0, _lib.C.byref(k)) \
+        == _lib.ERR_INVALID_ARGUMENT
+    assert F.lib().femtc_apply_kernel(0, F.F64, 26, 27, 0, _lib.C.byref(k)) \
+        == _lib.ERR_DEGREE_NOT_SUPPORTED
+    assert F.lib().femtc_apply_kernel(0, F.F64, 16, 17, 0, None) == \
+        _lib.ERR_INVALID_ARGUMENT
+
+
 def test_solver_create_rejects_bad_problems_without_touching_the_device():
     # argument checks run before any HIP call
     import ctypes as C_

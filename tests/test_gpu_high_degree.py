@@ -3,8 +3,9 @@ reference's dispatch ladder) through the C ABI against the CPU oracle:
 cell-local mass, stiffness and K+M, the reference's literal mass and
 stiffness, the dofmap gather with the atomic scatter, the affine-geometry
 variants and the reference-signature host shim.  The operator tests run
-twice, with the line kernels where they apply (degrees 9..19) and with the
-general kernel forced (femtc_set_high_degree_kernel).
+twice, with the line kernels where one exists for the shape (degrees 9..25,
+femtc_apply_kernel) and with the general kernel forced
+(femtc_set_high_degree_kernel).
 
 Bars: FP64 relative l2 <= 1e-12 against the oracle (whose literal operators
 match the compiled reference bit for bit at these degrees,
@@ -84,7 +85,7 @@ def test_cell_operator_f64_vs_oracle(op, P, Q, hd_kernel):
     assert rel(host(W), ref) <= TOL64, (op, P, Q)
 
 
-@pytest.mark.parametrize("P", [9, 16, 17, 19, 25])
+@pytest.mark.parametrize("P", [9, 16, 17, 19, 20, 22, 25])
 @pytest.mark.parametrize("op", ["mass", "stiffness", "kpm"])
 def test_cell_operator_f32_vs_oracle(op, P, hd_kernel):
     ncell, Q = 4, P + 1
@@ -97,7 +98,8 @@ def test_cell_operator_f32_vs_oracle(op, P, hd_kernel):
     assert err <= TOL32, (op, P)
 
 
-LINE_SHAPES = [(P, Q) for P in range(9, 20) for Q in (P + 1, P + 2, P + 3)]
+_OPS = {"mass": F.OP_MASS, "stiffness": F.OP_STIFFNESS, "kpm": F.OP_KPM}
+LINE_SHAPES = [(P, Q) for P in range(9, 26) for Q in (P + 1, P + 2, P + 3)]
 
 
 @pytest.mark.parametrize("P,Q", LINE_SHAPES)
@@ -109,6 +111,10 @@ def test_line_kernels_every_shape_vs_oracle(P, Q):
     print(f"line kernels P={P} Q={Q}", flush=True)
     N, tab, U, G, Dw = _case(P, Q, ncell, 11 * P + Q, table="random")
     for op in ("mass", "stiffness", "kpm"):
+        for dt in (F.F64, F.F32):
+            # the kernel each dtype runs is the one the query names (the
+            # FP32 line runs in test_cell_operator_f32_vs_oracle)
+            assert F.apply_kernel(_OPS[op], dt, P, Q) in ("line", "general")
         ref = O.cell_operator(op, N, Q, tab, U, G, Dw, ncell)
         W = torch.full((ncell * N ** 3,), -5.0, device="cuda",
                        dtype=torch.float64)
