@@ -45,10 +45,13 @@ constexpr int LINE_NMAX = 26;
 #define LINE_KPM64_LIVE 66
 #endif
 #ifndef LINE_PREFETCH
-#define LINE_PREFETCH 2
+#define LINE_PREFETCH 1
 #endif
 #ifndef LINE_PAIR
 #define LINE_PAIR 2
+#endif
+#ifndef LINE_CHUNK
+#define LINE_CHUNK 128
 #endif
 
 template <int N, int Q>
@@ -173,9 +176,11 @@ __device__ __forceinline__ float line_fma(float a, float b, float c)
 
 // Tables are read through a pointer in the constant address space, which
 // the compiler reads with scalar loads, laundered through an empty asm per
-// 64-byte chunk: each contraction then loads its table one chunk ahead of
-// the multiply-adds that use it, instead of the compiler hoisting a whole
-// table (up to 2 x 26 x 28 values) into scalar registers at once.
+// chunk of LINE_CHUNK bytes: each contraction then loads its table one
+// chunk ahead of the multiply-adds that use it, instead of the compiler
+// hoisting a whole table (up to 2 x 26 x 28 values) into scalar registers
+// at once.  128-byte chunks measured up to 8 % faster than 64-byte chunks
+// and than two 64-byte chunks in flight.
 template <typename T>
 using line_cptr = const __attribute__((address_space(4))) T*;
 
@@ -213,7 +218,7 @@ __device__ __forceinline__ void line_contract(const T* Mt, const T (&in)[NI],
                                               T (&out)[NO])
 {
   constexpr int TOT = NI * NO;
-  constexpr int K = 64 / (int)sizeof(T);
+  constexpr int K = LINE_CHUNK / (int)sizeof(T);
   constexpr int NCH = (TOT + K - 1) / K;
   // chunks in flight ahead of the one in use (a ring of D + 1 buffers)
   constexpr int D = LINE_PREFETCH < NCH ? LINE_PREFETCH : NCH;
@@ -228,52 +233,59 @@ __device__ __forceinline__ void line_contract(const T* Mt, const T (&in)[NI],
       if (k * K + j < TOT)
         b[j] = M[j];
   };
+  auto product = [&](const T (&cur)[K], int k, int j) {
+    const int f = k * K + j;
+    if (f < TOT)
+    {
+      if constexpr (!BWD)
+      {
+        const int i = f / NO, o = f - (f / NO) * NO;
+        out[o] = (!ACC && i == 0) ? cur[j] * in[i]
+                                  : line_fma(cur[j], in[i], out[o]);
+      }
+      else
+      {
+        const int o = f / NI, i = f - (f / NI) * NI;
+        out[o] = (!ACC && i == 0) ? cur[j] * in[i]
+                                  : line_fma(cur[j], in[i], out[o]);
+      }
+    }
+  };
+  // pin: the product is computed before whatever follows this asm
+  auto pin = [&](int k, int j, bool always) {
+    const int f = k * K + j;
+    if (f < TOT)
+    {
+      if constexpr (!BWD)
+        asm volatile("" : "+v"(out[f - (f / NO) * NO]));
+      else if (always || f - (f / NI) * NI == NI - 1 || j == K - 1)
+        asm volatile("" : "+v"(out[f / NI]));
+    }
+  };
   line_static_for<D>([&](auto kc) {
     constexpr int k = decltype(kc)::value;
     load(k, ring[k]);
   });
   line_static_for<NCH>([&](auto kc) {
     constexpr int k = decltype(kc)::value;
-    // nothing moves across: the loads of chunk k + D stay with the
-    // multiply-adds of chunk k
     __builtin_amdgcn_sched_barrier(0);
+    const T (&cur)[K] = ring[k % (D + 1)];
+    // Scalar loads return out of order, so the wait before this chunk's
+    // first product waits for every scalar load in flight.  The loads of
+    // chunk k + D are issued after that product, not before, so that wait
+    // does not include them.
+    product(cur, k, 0);
+    pin(k, 0, true);
     if constexpr (k + D < NCH)
       load(k + D, ring[(k + D) % (D + 1)]);
-    const T (&cur)[K] = ring[k % (D + 1)];
 #pragma unroll
-    for (int j = 0; j < K; ++j)
-    {
-      const int f = k * K + j;
-      if (f < TOT)
-      {
-        if constexpr (!BWD)
-        {
-          const int i = f / NO, o = f - (f / NO) * NO;
-          out[o] = (!ACC && i == 0) ? cur[j] * in[i]
-                                    : line_fma(cur[j], in[i], out[o]);
-        }
-        else
-        {
-          const int o = f / NI, i = f - (f / NI) * NI;
-          out[o] = (!ACC && i == 0) ? cur[j] * in[i]
-                                    : line_fma(cur[j], in[i], out[o]);
-        }
-      }
-    }
+    for (int j = 1; j < K; ++j)
+      product(cur, k, j);
     // the chunk's products are done here, before the loads issued with
     // the next chunk (ordered behind this asm)
 #pragma unroll
-    for (int j = 0; j < K; ++j)
-    {
-      const int f = k * K + j;
-      if (f < TOT)
-      {
-        if constexpr (!BWD)
-          asm volatile("" : "+v"(out[f - (f / NO) * NO]));
-        else if (f - (f / NI) * NI == NI - 1 || j == K - 1)
-          asm volatile("" : "+v"(out[f / NI]));
-      }
-    }
+    for (int j = 1; j < K; ++j)
+      pin(k, j, false);
   });
 }
 
