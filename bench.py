@@ -542,6 +542,49 @@ def bench_high_degree(torch, F, reps=10):
             del t, u, g, d, w
         del U, G, D
         torch.cuda.empty_cache()
+    # affine cells (7 geometry values per cell, computed into G(q) in the
+    # kernel): the continuous stiffness apply on an n^3 mesh of about 2e7
+    # global dofs, FP64
+    for P in (9, 16, 25):
+        Q = P + 1
+        n = max(2, round((2.0e7 ** (1 / 3) - 1) / P))
+        C = n ** 3
+        ng = (n * P + 1) ** 3
+        dm = torch.empty(C * (P + 1) ** 3, device="cuda", dtype=torch.int32)
+        F.dofmap_structured(n, n, n, P, dm)
+        J = _uniform(11 + P, 9 * C, -1, 1) * 0.1
+        J = torch.from_numpy((J.reshape(C, 3, 3) + np.eye(3)).ravel()).cuda()
+        cg = torch.empty(C * 7, device="cuda", dtype=torch.float64)
+        F.geometry_cell(J, cg)
+        tw = torch.from_numpy(F.basis_tables_weights(P, Q)).cuda()
+        x = torch.from_numpy(_uniform(12 + P, ng, -1, 1)).cuda()
+        y = torch.empty_like(x)
+
+        def run_aff():
+            F.global_apply_affine(F.OP_STIFFNESS, tw, x, y, dm, C, P, Q, cg)
+
+        def time_aff():
+            for _ in range(3):
+                run_aff()
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                run_aff()
+            e1.record()
+            torch.cuda.synchronize()
+            return e0.elapsed_time(e1) / reps
+        ms = time_aff()
+        with F.high_degree_kernel(F.HIGH_DEGREE_GENERAL):
+            ms_gen = time_aff()
+        out[f"P{P}_stiffness_affine_global_f64"] = {
+            "mesh": f"{n}^3", "cells": C, "global_dofs": ng,
+            "us": ms * 1e3, "gdofs": ng / (ms * 1e-3) / 1e9,
+            "general_us": ms_gen * 1e3,
+            "general_gdofs": ng / (ms_gen * 1e-3) / 1e9,
+            "speedup_vs_general": ms_gen / ms}
+        del dm, J, cg, tw, x, y
+        torch.cuda.empty_cache()
     return out
 
 

@@ -2,11 +2,11 @@
 // 9..25, the top of the reference's dispatch ladder, src/mass.hpp:52-138 and
 // src/stiffness.hpp:102-188), Q in {N, N+1, N+2}, FP64 and FP32, cell-local
 // or gathered through a dofmap with an atomic scatter.  Mass, stiffness and
-// K+M in correct mode go to the line kernels (apply_line.hpp) where one
-// exists for (dtype, op, N, Q) -- N = 10..22 in FP64 stiffness, up to 26
-// for the rest (line_fits) -- unless femtc_set_high_degree_kernel asks for
-// this one; the reference modes, the affine-geometry operators and the
-// shapes without a line kernel run here.
+// K+M in correct mode (stored or affine geometry) go to the line kernels
+// (apply_line.hpp) where one exists for (dtype, op, N, Q) -- N = 10..22 in
+// FP64 stiffness, up to 26 for the rest (line_fits) -- unless
+// femtc_set_high_degree_kernel asks for this one; the reference modes and
+// the shapes without a line kernel run here.
 //
 // The fused kernels hold a whole cell per wave in registers and LDS, which
 // stops at N = 9.  Here one workgroup (256 to 768 threads) takes one cell at a time
@@ -418,13 +418,13 @@ bool line_disabled()
 }
 
 // The line kernels (apply_line.hpp) for N = 10..26 in correct mode: mass,
-// stiffness and K+M with per-point geometry, cell-local or gathered.
+// stiffness and K+M with per-point geometry or on affine cells, cell-local
+// or gathered.
 // Returns FEMTC_ERR_INVALID_ARGUMENT when no line kernel covers the request
 // (the general kernel then runs).
 int line_launch(const ApplyReq& r)
 {
-  if (r.reference_mode || (r.op != OP_MASS && r.op != OP_STIFF &&
-                           r.op != OP_KPM))
+  if (r.reference_mode || r.op == OP_STIFF_LIT)
     return FEMTC_ERR_INVALID_ARGUMENT;
   if (line_disabled())
     return FEMTC_ERR_INVALID_ARGUMENT;

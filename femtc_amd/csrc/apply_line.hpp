@@ -1,8 +1,9 @@
 // Line kernels for the high degrees (the top of the reference's dispatch
 // ladder, src/mass.hpp:52-138 and src/stiffness.hpp:102-188): cell-local or
-// gathered mass, stiffness and K+M in correct mode, N = LINE_NMIN..LINE_NMAX
-// dofs per direction, Q in {N, N+1, N+2}, FP64 and FP32, for the shapes
-// whose registers and LDS fit (line_fits).
+// gathered mass, stiffness and K+M in correct mode, with per-point geometry
+// or on affine cells (7 values per cell), N = LINE_NMIN..LINE_NMAX dofs per
+// direction, Q in {N, N+1, N+2}, FP64 and FP32, for the shapes whose
+// registers and LDS fit (line_fits).
 //
 // One workgroup takes one cell at a time.  Every contraction is done by
 // threads that own a whole line of the tensor along the contracted axis,
@@ -66,6 +67,12 @@ struct LineShape
   static constexpr int SLICE = Q * NN > QQ * NP ? Q * NN : QQ * NP;
 };
 
+// The stored-geometry operator an affine one computes (OP_*_AFF -> OP_*).
+constexpr int line_base_op(int op)
+{
+  return op >= OP_MASS_AFF ? op - OP_MASS_AFF : op;
+}
+
 // Waves per SIMD a cell's registers are sized for: 3 (168 VGPRs), or 2
 // (256 VGPRs) for FP64 stiffness and K+M when the values live at once --
 // the three gradient lines (and u) at Q points plus one line of N -- do not
@@ -75,9 +82,10 @@ template <typename T, int OP, int N, int Q>
 constexpr int line_want()
 {
   constexpr bool f64 = sizeof(T) == 8;
-  return (f64 && OP == OP_STIFF && 3 * Q + N > LINE_STIFF64_LIVE)  ? 2
-         : (f64 && OP == OP_KPM && 4 * Q + N > LINE_KPM64_LIVE) ? 2
-                                                                 : 3;
+  constexpr int B = line_base_op(OP);
+  return (f64 && B == OP_STIFF && 3 * Q + N > LINE_STIFF64_LIVE)  ? 2
+         : (f64 && B == OP_KPM && 4 * Q + N > LINE_KPM64_LIVE) ? 2
+                                                                : 3;
 }
 
 // LDS slices per cell: two alternating slices (one barrier per transpose)
@@ -108,8 +116,9 @@ constexpr size_t line_cell_lds()
 template <typename T, int OP, int N, int Q>
 constexpr int line_vgprs()
 {
-  constexpr int live = OP == OP_MASS ? Q + N : OP == OP_STIFF ? 3 * Q + N
-                                                              : 4 * Q + N;
+  constexpr int B = line_base_op(OP);
+  constexpr int live = B == OP_MASS ? Q + N : B == OP_STIFF ? 3 * Q + N
+                                                            : 4 * Q + N;
   return live * (int)(sizeof(T) / 4) + 24;
 }
 
@@ -310,25 +319,57 @@ __device__ __forceinline__ void line_bwd(const T* M, const T (&in)[NI],
 template <typename T, int OP, int N, int Q, int NA, int NU>
 __device__ __forceinline__ void line_pointwise(const T* __restrict__ G,
                                                const T* __restrict__ detJw,
+                                               const T* __restrict__ tab,
                                                int64_t cell, int tq,
                                                T (&DA)[NA], T (&DB)[NA],
                                                T (&DC)[NA], T (&U)[NU])
 {
-  constexpr bool FULL = OP == OP_STIFF || OP == OP_KPM;
-  constexpr bool MT = OP == OP_MASS || OP == OP_KPM;
+  constexpr int B = line_base_op(OP);
+  constexpr bool AFF = OP >= OP_MASS_AFF;
+  constexpr bool FULL = B == OP_STIFF || B == OP_KPM;
+  constexpr bool MT = B == OP_MASS || B == OP_KPM;
   constexpr int GV = (FULL ? 6 : 0) + (MT ? 1 : 0);
   const int64_t p0 = cell * (int64_t)(Q * Q * Q) + (int64_t)tq * Q;
+  // affine cells: the cell's 7 values (detJ J^-1 J^-T packed as G, detJ;
+  // the same for every lane of the wave) times the tensor-product Gauss
+  // weights w[qa] w[qb] w[qc] (after the two tables)
+  T cg[AFF ? CELLGEO : 1];
+  T wab = T(1);
+  const T* const wts = tab + 2 * N * Q;
+  if constexpr (AFF)
+  {
+#pragma unroll
+    for (int k = 0; k < CELLGEO; ++k)
+      cg[k] = G[cell * CELLGEO + k];
+    const int qa = tq / Q;
+    wab = wts[qa] * wts[tq - qa * Q];
+  }
   T gcur[GV], gnxt[GV];
   auto load = [&](int qc, T (&gb)[GV]) {
-    if constexpr (FULL)
+    if constexpr (AFF)
     {
-      const T* g = G + (p0 + qc) * 6;
+      const T wq = wab * wts[qc];
+      if constexpr (FULL)
+      {
 #pragma unroll
-      for (int k = 0; k < 6; ++k)
-        gb[k] = g[k];
+        for (int k = 0; k < 6; ++k)
+          gb[k] = wq * cg[k];
+      }
+      if constexpr (MT)
+        gb[GV - 1] = wq * cg[6];
     }
-    if constexpr (MT)
-      gb[GV - 1] = detJw[p0 + qc];
+    else
+    {
+      if constexpr (FULL)
+      {
+        const T* g = G + (p0 + qc) * 6;
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+          gb[k] = g[k];
+      }
+      if constexpr (MT)
+        gb[GV - 1] = detJw[p0 + qc];
+    }
   };
   load(0, gcur);
   line_static_for<Q>([&](auto qcc) {
@@ -368,8 +409,9 @@ __global__ void __launch_bounds__((line_threads<T, OP, N, Q>()))
                const int32_t* __restrict__ dofmap, int64_t C)
 {
   using S = LineShape<N, Q>;
-  constexpr bool FULL = OP == OP_STIFF || OP == OP_KPM;
-  constexpr bool MT = OP == OP_MASS || OP == OP_KPM;
+  constexpr int OPB = line_base_op(OP);
+  constexpr bool FULL = OPB == OP_STIFF || OPB == OP_KPM;
+  constexpr bool MT = OPB == OP_MASS || OPB == OP_KPM;
   static_assert(FULL || MT, "mass, stiffness or K+M");
   constexpr int NN = S::NN, QN = S::QN, QQ = S::QQ, NP = S::NP;
   // transposes per cell: A0 A1 B0 B1 B2 D1 D2 D3 E1 E2, or A0 B0 D3 E2;
@@ -530,7 +572,7 @@ __global__ void __launch_bounds__((line_threads<T, OP, N, Q>()))
       __syncthreads();
       r_ab_c(4, l);
       line_fwd<T, N, Q>(phi, l, DA);
-      line_pointwise<T, OP, N, Q>(G, detJw, cell, tab_, DA, DB, DC, U);
+      line_pointwise<T, OP, N, Q>(G, detJw, tab, cell, tab_, DA, DB, DC, U);
       T D1[N], D2[N], D3[N];
       line_bwd<T, Q, N>(dph, DC, D3);
       if constexpr (MT)
@@ -594,7 +636,7 @@ __global__ void __launch_bounds__((line_threads<T, OP, N, Q>()))
       __syncthreads();
       r_ab_c(4, l);
       line_fwd<T, N, Q>(phi, l, DA);
-      line_pointwise<T, OP, N, Q>(G, detJw, cell, tab_, DA, DB, DC, U);
+      line_pointwise<T, OP, N, Q>(G, detJw, tab, cell, tab_, DA, DB, DC, U);
       // B_c along c, each result to LDS as soon as it is complete
       T D1[N], D2[N], D3[N];
       line_bwd<T, Q, N>(dph, DC, D3);
@@ -642,7 +684,8 @@ __global__ void __launch_bounds__((line_threads<T, OP, N, Q>()))
       r_ab_c(1, l);
       line_fwd<T, N, Q>(phi, l, U);
       T none[1];
-      line_pointwise<T, OP, N, Q>(G, detJw, cell, tab_, none, none, none, U);
+      line_pointwise<T, OP, N, Q>(G, detJw, tab, cell, tab_, none, none, none,
+                                  U);
       line_bwd<T, Q, N>(phi, U, D3);
       wb();
       w_ab(2, D3);
@@ -716,6 +759,11 @@ int line_launch_q(const ApplyReq& r, int max_lds)
   case OP_MASS: return line_launch_fit<T, OP_MASS, N, Q>(r, max_lds);
   case OP_STIFF: return line_launch_fit<T, OP_STIFF, N, Q>(r, max_lds);
   case OP_KPM: return line_launch_fit<T, OP_KPM, N, Q>(r, max_lds);
+  case OP_MASS_AFF:
+    return line_launch_fit<T, OP_MASS_AFF, N, Q>(r, max_lds);
+  case OP_STIFF_AFF:
+    return line_launch_fit<T, OP_STIFF_AFF, N, Q>(r, max_lds);
+  case OP_KPM_AFF: return line_launch_fit<T, OP_KPM_AFF, N, Q>(r, max_lds);
   }
   return FEMTC_ERR_INVALID_ARGUMENT;
 }

@@ -194,8 +194,8 @@ int femtc_fill_zero(void* y, int64_t n, int dtype, void* stream);
 /* Kernel choice for degrees above FEMTC_MAX_FUSED_DEGREE (no counterpart
  * in the reference, whose ladder compiles one CPU loop per degree).
  * FEMTC_HIGH_DEGREE_AUTO: the line kernels where they apply (correct mode,
- * mass / stiffness / K+M with per-point geometry, N = degree+1 in 10..26
- * where a cell's registers and LDS fit: FP64 stiffness and K+M up to
+ * mass / stiffness / K+M with per-point or affine geometry, N = degree+1 in
+ * 10..26 where a cell's registers and LDS fit: FP64 stiffness and K+M up to
  * N = 21..22, the rest up to 26), else the general kernel;
  * FEMTC_HIGH_DEGREE_GENERAL: the general kernel always.  Process-wide;
  * the previous setting goes to *previous (may be NULL).  Another mode:
@@ -208,8 +208,9 @@ int femtc_set_high_degree_kernel(int mode, int* previous);
  * FEMTC_OP_*, dtype, degree, nq, mode FEMTC_MODE_*) under the current
  * setting above (no counterpart in the reference): *kernel =
  * FEMTC_KERNEL_FUSED (degrees 1..FEMTC_MAX_FUSED_DEGREE),
- * FEMTC_KERNEL_LINE or FEMTC_KERNEL_GENERAL.  Assumes the 160 KB of LDS
- * per workgroup of gfx950; no device call. */
+ * FEMTC_KERNEL_LINE or FEMTC_KERNEL_GENERAL.  The affine-geometry applies
+ * (femtc_global_apply_affine) follow the same choice.  Assumes the 160 KB
+ * of LDS per workgroup of gfx950; no device call. */
 #define FEMTC_KERNEL_FUSED 0
 #define FEMTC_KERNEL_GENERAL 1
 #define FEMTC_KERNEL_LINE 2

@@ -219,9 +219,10 @@ def test_global_operator_vs_oracle(op, nx, ny, nz, P, Q, hd_kernel):
     assert rel(host(y), ref + y0) <= TOL64
 
 
-@pytest.mark.parametrize("P,Q", [(9, 10), (12, 14), (17, 18), (23, 25)])
+@pytest.mark.parametrize("P,Q", [(9, 10), (12, 14), (16, 17), (17, 18),
+                                 (20, 22), (23, 25), (25, 26)])
 @pytest.mark.parametrize("op", [F.OP_MASS, F.OP_STIFFNESS, F.OP_KPM])
-def test_affine_operator_vs_oracle(op, P, Q):
+def test_affine_operator_vs_oracle(op, P, Q, hd_kernel):
     nx, ny, nz = 2, 1, 2
     C, J, G, Dw, dm, ng = _box(nx, ny, nz, P, Q, 70 + P)
     x = O.uniform(71, ng)
