@@ -26,7 +26,8 @@
 //
 // Each arrow between owners is one transpose through LDS.  Two LDS slices
 // alternate, so one barrier per transpose orders both the slice's write
-// after its last read (two transposes back) and its read after its write.
+// after its last read (two transposes back) and its read after its write;
+// cells too large for two slices use one, with two barriers per transpose.
 #pragma once
 
 #include <algorithm>
@@ -104,11 +105,6 @@ constexpr size_t line_cell_lds()
   return line_nbuf<T, N, Q>() * (size_t)LineShape<N, Q>::SLICE * sizeof(T);
 }
 
-// Cells per workgroup: 2 when one cell takes 5 to 7 waves, its registers
-// fit 3 waves per SIMD and two cells' LDS fits the 160 KB of a CU
-// (measured: a second workgroup of 5 waves does not become resident beside
-// the first, so one workgroup of two cells doubles the waves per CU), else
-// 1.
 // Registers a cell's values need: the values live at once (mass: a line of
 // Q and one of N; stiffness: three of Q and one of N; K+M: four of Q and one
 // of N) plus about 24 of addresses and temporaries (the compiler's counts
@@ -128,6 +124,11 @@ constexpr int line_budget(int w)
   return w <= 2 ? 256 : (512 / w) / 8 * 8;
 }
 
+// Cells per workgroup: 2 when one cell takes 5 to 7 waves, its registers
+// fit 3 waves per SIMD and two cells' LDS fits the 160 KB of a CU
+// (measured: a second workgroup of 5 waves does not become resident beside
+// the first, so one workgroup of two cells doubles the waves per CU), else
+// 1.
 template <typename T, int OP, int N, int Q>
 constexpr int line_cpb()
 {
@@ -172,7 +173,6 @@ constexpr bool line_fits()
          line_cell_lds<T, N, Q>() <= 160 * 1024 &&
          line_threads<T, OP, N, Q>() <= 1024;
 }
-
 
 __device__ __forceinline__ double line_fma(double a, double b, double c)
 {
