@@ -217,6 +217,12 @@ def test_global_operator_vs_oracle(op, nx, ny, nz, P, Q, hd_kernel):
     F.global_apply(op, dev(tab), dev(x), y, ddm, C, P, Q, G=dev(G),
                    detJw=dev(Dw), accumulate=True)
     assert rel(host(y), ref + y0) <= TOL64
+    # FP32: the gather and the FP32 atomic scatter
+    f32 = torch.float32
+    y = torch.full((ng,), 2.0, device="cuda", dtype=f32)
+    F.global_apply(op, dev(tab, f32), dev(x, f32), y, ddm, C, P, Q,
+                   G=dev(G, f32), detJw=dev(Dw, f32))
+    assert rel(host(y), ref) <= TOL32
 
 
 @pytest.mark.parametrize("P,Q", [(9, 10), (12, 14), (16, 17), (17, 18),
