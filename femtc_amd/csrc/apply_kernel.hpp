@@ -5,11 +5,11 @@
 // registers, with the table entries as wave-uniform (scalar) operands.
 // Between contractions along different axes the partial tensors pass through
 // LDS.  A workgroup is one wave holding CPB = 64/(Q*Q) cells (Q <= 8), or the
-// fewest waves that hold one cell (Q > 8), so the stage barriers are cheap
-// and the workgroups resident on a CU progress independently, hiding each
-// other's memory latency.  The FP32 cell pass of the assembled scatter at
-// Q = 6, 7 takes 4 waves of 7 and 5 cells instead (assembled_cells), for
-// busy lanes.
+// fewest waves that hold one cell (Q = 10, 11) or three (Q = 9: 243 of 256
+// lanes), so the stage barriers are cheap and the workgroups resident on a
+// CU progress independently, hiding each other's memory latency.  The FP32
+// cell pass of the assembled scatter at Q = 6, 7 takes 4 waves of 7 and 5
+// cells instead (assembled_cells), for busy lanes.
 //
 // Memory access.  Every stream is read with as few, as wide and as
 // contiguous wave instructions as possible (the vector-memory address path
@@ -71,7 +71,7 @@ namespace femtc
 template <int Q>
 constexpr int cells_per_block()
 {
-  return Q * Q <= 64 ? 64 / (Q * Q) : 1;
+  return direct_cells(Q);
 }
 
 template <int Q>
@@ -548,8 +548,9 @@ apply_body(const T* __restrict__ tab, const T* __restrict__ x,
       cell0 = blk * CPB;
       ncells_blk = num_cells - cell0 < CPB ? num_cells - cell0 : (long long)CPB;
     }
-    // one cell per group (Q >= 6): the cell is wave-uniform, so its index
-    // arithmetic (64-bit products and divisions) runs on the scalar unit
+    // one cell per group (Q = 6..8, 10, 11): the cell is wave-uniform, so
+    // its index arithmetic (64-bit products and divisions) runs on the
+    // scalar unit
     const long long cell = CPB == 1 ? cell0 : cell0 + lc;
     const bool live = lane_ok && lc < ncells_blk;
 

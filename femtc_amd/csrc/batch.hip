@@ -55,7 +55,7 @@ BatchLauncher batch_launcher(int N)
   return nullptr;
 }
 
-int cells_per_wg(int nq) { return nq * nq <= 64 ? 64 / (nq * nq) : 1; }
+int cells_per_wg(int nq) { return direct_cells(nq); }
 
 constexpr int NSTREAMS = 4;
 

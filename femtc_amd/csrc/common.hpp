@@ -95,18 +95,27 @@ constexpr int SCATTER_STRUCT = -2;
 // BW template value of the structured z-run scatter (ApplyReq::zrun).
 constexpr int SCATTER_ZRUN = -3;
 
+// Cells per workgroup of the fused cell kernel (Q^2 threads per cell):
+// one wave of whole cells up to Q = 8.  At Q = 9 one cell in a 2-wave
+// workgroup would leave 47 of its 128 lanes idle; three cells take 243 of
+// 4 waves' 256.  At Q = 10, 11 one cell keeps 100 and 121 of 128 busy.
+constexpr int direct_cells(int Q)
+{
+  return Q * Q <= 64 ? 64 / (Q * Q) : Q == 9 ? 3 : 1;
+}
+
 // Cells per workgroup of the assembled scatter's cell pass (esize: bytes
-// per value).  One wave of whole cells (Q^2 threads each) leaves lanes idle
-// at Q = 6 and 7 (36 and 49 of 64 busy); for FP32 the workgroup there takes
-// 7 and 5 cells over 4 waves (252/256, 245/256 lanes), which made the ALU-
-// bound FP32 cell pass faster.  FP64 (and the direct and blocked scatters,
-// whose atomics measured 4-19 % slower from multi-wave workgroups) keep
-// one-wave workgroups.
+// per value).  One wave of whole cells leaves lanes idle at Q = 6 and 7 (36
+// and 49 of 64 busy); for FP32 the workgroup there takes 7 and 5 cells over
+// 4 waves (252/256, 245/256 lanes), which made the ALU-bound FP32 cell pass
+// faster.  FP64 (and the direct and blocked scatters, whose atomics
+// measured 4-19 % slower from multi-wave workgroups) keep one-wave
+// workgroups there.
 constexpr int assembled_cells(int Q, int esize)
 {
   return esize == 4 && Q == 6   ? 7
          : esize == 4 && Q == 7 ? 5
-                                : (Q * Q <= 64 ? 64 / (Q * Q) : 1);
+                                : direct_cells(Q);
 }
 
 // One problem of a batched launch (device-resident descriptor).

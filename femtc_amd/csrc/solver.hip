@@ -946,7 +946,7 @@ int64_t energy_partials(int64_t C, int Q, bool assembled, int dtype)
   const int64_t Qq = (int64_t)Q * Q;
   const int64_t cpg =
       assembled ? assembled_cells(Q, dtype == FEMTC_F64 ? 8 : 4)
-                : (Qq <= 64 ? 64 / Qq : 1);
+                : direct_cells(Q);
   const int64_t waves = (cpg * Qq + 63) / 64;
   return (C + cpg - 1) / cpg * waves;
 }
