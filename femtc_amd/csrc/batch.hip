@@ -369,8 +369,10 @@ int run_all(femtc_batch* b, hipStream_t s)
 
 // The outputs the batch clears before its applies: every continuous item
 // except those of groups on the assembled scatter, whose second pass
-// overwrites y.  Synchronous; called whenever a group's scatter changes.
-int upload_fills(femtc_batch* b)
+// overwrites y (all: every continuous item, for the autotuner, which times
+// each candidate with the clears it needs).  Synchronous; called whenever a
+// group's scatter changes.
+int upload_fills(femtc_batch* b, bool all = false)
 {
   std::vector<FillDesc> fills = b->empty_fill;
   b->max_empty_fill = 0;
@@ -381,7 +383,7 @@ int upload_fills(femtc_batch* b)
     g.fill_first = fills.size();
     g.nfill = 0;
     g.max_fill = 0;
-    if (g.per_problem == 2)
+    if (g.per_problem == 2 && !all)
       continue;
     for (const FillDesc& f : g.fill)
     {
@@ -621,7 +623,9 @@ int femtc_batch_autotune(femtc_batch* b, int reps)
   hipEvent_t e0, e1;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
     return fail(FEMTC_ERR_HIP, "event creation failed");
-  int st = FEMTC_OK;
+  // every group's clears on the device, so that a candidate that needs
+  // them (all but the assembled scatter) is timed with them
+  int st = upload_fills(b, true);
   // the whole batch a few times first: a GPU coming out of idle runs its
   // first launches slower, which would favour the candidates timed last
   for (int i = 0; i < 4 && st == FEMTC_OK; ++i)
@@ -655,10 +659,18 @@ int femtc_batch_autotune(femtc_batch* b, int reps)
                         : k == -3 ? 4
                                   : 0;
         t.req.per_wg = k > 0 ? k : 1;
-        st = launch_group(t, b->stream); // warm-up
+        // the candidate as the batch runs it: y cleared first unless the
+        // assembled scatter overwrites it
+        auto launch = [&]() {
+          if (t.per_problem != 2)
+            launch_fill(b->d_fill + t.fill_first, t.nfill, t.max_fill,
+                        b->stream);
+          return launch_group(t, b->stream);
+        };
+        st = launch(); // warm-up
         (void)hipEventRecord(e0, b->stream);
         for (int i = 0; i < reps && st == FEMTC_OK; ++i)
-          st = launch_group(t, b->stream);
+          st = launch();
         (void)hipEventRecord(e1, b->stream);
         if (st != FEMTC_OK || hipEventSynchronize(e1) != hipSuccess)
           break;
