@@ -1,10 +1,11 @@
 // Batched operator applies: many independent problems (meshes) of uneven
 // size, degree, operator and precision in a few launches.  Problems that
 // share (op, dtype, degree, nq, gather) form a group and run as ONE launch
-// of apply_batch (each workgroup looks its problem up by block index); the
-// groups run on up to four forked streams so small groups overlap; the whole
-// sequence can be captured once into a hipGraph and replayed.  The
-// per-group knob (blocks per workgroup) is chosen by femtc_batch_autotune.
+// of apply_batch (each workgroup looks its problem up by block index) or as
+// one launch per problem, as femtc_batch_autotune finds faster; the
+// launches are spread over four forked streams, longest first, so that
+// small ones overlap; the whole sequence can be captured once into a
+// hipGraph and replayed.
 #include <algorithm>
 #include <map>
 #include <tuple>
@@ -74,8 +75,7 @@ struct Group
   std::vector<BatchDesc> host;
   std::vector<Extra> extra; // parallel to host
   std::vector<FillDesc> fill; // continuous groups: y of each item
-  // the group's clears in the device fill array (none on the assembled
-  // scatter, whose second pass overwrites y)
+  // the group's clears in the device fill array while the autotuner runs
   size_t fill_first = 0;
   int nfill = 0;
   long long max_fill = 0;
@@ -89,6 +89,17 @@ struct Group
   // when that mode is first selected or timed
   BatchDesc* zdev = nullptr;
   long long zseg_total = 0;
+  // time of one launch of the group in its mode, as the autotuner measured
+  // it (0: not measured; the scheduler then estimates it from the bytes)
+  double est_ms = 0.0;
+};
+
+// One unit of the batch's schedule: a group's fused launch (item < 0) or
+// one problem of a group that runs one launch per problem.
+struct Task
+{
+  int g;
+  int item;
 };
 
 // Lattice buffers of a group's problems (allocated once).
@@ -173,6 +184,8 @@ void free_lattices(Group& g)
     }
 }
 
+int launch_item(const Group& g, size_t i, hipStream_t s);
+
 // Launch a group: fused, or one launch per problem (atomic, assembled or
 // z-run scatter).
 int launch_group(const Group& g, hipStream_t s)
@@ -193,6 +206,17 @@ int launch_group(const Group& g, hipStream_t s)
     return f ? f(r) : FEMTC_ERR_DEGREE_NOT_SUPPORTED;
   }
   for (size_t i = 0; i < g.host.size(); ++i)
+  {
+    const int st = launch_item(g, i, s);
+    if (st != FEMTC_OK)
+      return st;
+  }
+  return FEMTC_OK;
+}
+
+// Problem i of a group that runs one launch per problem.
+int launch_item(const Group& g, size_t i, hipStream_t s)
+{
   {
     const BatchDesc& d = g.host[i];
     ApplyReq a;
@@ -227,11 +251,8 @@ int launch_group(const Group& g, hipStream_t s)
     a.dofmap = d.dofmap;
     a.num_cells = d.num_cells;
     a.stream = s;
-    const int st = run(a);
-    if (st != FEMTC_OK)
-      return st;
+    return run(a);
   }
-  return FEMTC_OK;
 }
 
 } // namespace
@@ -256,6 +277,12 @@ struct femtc_batch
   hipGraphExec_t exec = nullptr;
   int64_t total_cells = 0;
   int64_t total_dofs = 0;
+  // the schedule: each side stream's tasks in launch order, and the range
+  // of d_fill it clears before its first task
+  std::vector<Task> tasks[NSTREAMS];
+  size_t sfill_first[NSTREAMS] = {};
+  int sfill_n[NSTREAMS] = {};
+  long long sfill_max[NSTREAMS] = {};
 };
 
 namespace
@@ -337,9 +364,10 @@ void launch_fill(const FillDesc* d, int n, long long max_n, hipStream_t s)
                      s, d);
 }
 
-// Launch every group once, forking the groups over the side streams and
-// joining back into b->stream; each group clears its continuous outputs on
-// its own stream first, so the clears overlap other groups' applies.
+// Launch every group once: the schedule's tasks on the side streams, forked
+// from and joined back into b->stream; each side stream first clears the
+// continuous outputs of its own tasks, so the clears overlap the other
+// streams' applies.
 int run_all(femtc_batch* b, hipStream_t s)
 {
   launch_fill(b->d_fill, (int)b->empty_fill.size(), b->max_empty_fill, s);
@@ -348,15 +376,19 @@ int run_all(femtc_batch* b, hipStream_t s)
   for (int i = 0; i < NSTREAMS; ++i)
     if (hipStreamWaitEvent(b->side[i], b->ev_fork, 0) != hipSuccess)
       return fail(FEMTC_ERR_HIP, "batch: stream wait failed");
-  for (size_t g = 0; g < b->groups.size(); ++g)
+  for (int i = 0; i < NSTREAMS; ++i)
   {
-    const Group& grp = b->groups[g];
-    launch_fill(b->d_fill + grp.fill_first, grp.nfill, grp.max_fill,
-                b->side[g % NSTREAMS]);
-    const int st = launch_group(grp, b->side[g % NSTREAMS]);
-    if (st != FEMTC_OK)
-      return fail(st, std::string("batch launch: ") +
-                          hipGetErrorString(hipGetLastError()));
+    launch_fill(b->d_fill + b->sfill_first[i], b->sfill_n[i],
+                b->sfill_max[i], b->side[i]);
+    for (const Task& t : b->tasks[i])
+    {
+      const Group& grp = b->groups[t.g];
+      const int st = t.item < 0 ? launch_group(grp, b->side[i])
+                                : launch_item(grp, (size_t)t.item, b->side[i]);
+      if (st != FEMTC_OK)
+        return fail(st, std::string("batch launch: ") +
+                            hipGetErrorString(hipGetLastError()));
+    }
   }
   for (int i = 0; i < NSTREAMS; ++i)
   {
@@ -367,24 +399,25 @@ int run_all(femtc_batch* b, hipStream_t s)
   return FEMTC_OK;
 }
 
-// The outputs the batch clears before its applies: every continuous item
-// except those of groups on the assembled scatter, whose second pass
-// overwrites y (all: every continuous item, for the autotuner, which times
-// each candidate with the clears it needs).  Synchronous; called whenever a
-// group's scatter changes.
-int upload_fills(femtc_batch* b, bool all = false)
+int upload(femtc_batch* b, const std::vector<FillDesc>& fills)
+{
+  if (!fills.empty() &&
+      hipMemcpy(b->d_fill, fills.data(), fills.size() * sizeof(FillDesc),
+                hipMemcpyHostToDevice) != hipSuccess)
+    return fail(FEMTC_ERR_HIP, "batch descriptor upload failed");
+  return FEMTC_OK;
+}
+
+// Every continuous item's clear, group by group (fill_first, nfill), for the
+// autotuner, which times each candidate with the clears it needs.
+// Synchronous.
+int upload_group_fills(femtc_batch* b)
 {
   std::vector<FillDesc> fills = b->empty_fill;
-  b->max_empty_fill = 0;
-  for (const FillDesc& f : fills)
-    b->max_empty_fill = std::max(b->max_empty_fill, f.n);
   for (Group& g : b->groups)
   {
     g.fill_first = fills.size();
-    g.nfill = 0;
     g.max_fill = 0;
-    if (g.per_problem == 2 && !all)
-      continue;
     for (const FillDesc& f : g.fill)
     {
       fills.push_back(f);
@@ -392,11 +425,94 @@ int upload_fills(femtc_batch* b, bool all = false)
     }
     g.nfill = (int)g.fill.size();
   }
-  if (!fills.empty() &&
-      hipMemcpy(b->d_fill, fills.data(), fills.size() * sizeof(FillDesc),
-                hipMemcpyHostToDevice) != hipSuccess)
-    return fail(FEMTC_ERR_HIP, "batch descriptor upload failed");
-  return FEMTC_OK;
+  return upload(b, fills);
+}
+
+// Time of one launch of a group in its mode: the autotuner's measurement,
+// else its algorithmic bytes at 4 TB/s.
+double group_estimate(const Group& g)
+{
+  if (g.est_ms > 0.0)
+    return g.est_ms;
+  const double es = g.req.dtype == FEMTC_F64 ? 8.0 : 4.0;
+  const double N3 = (double)g.req.N * g.req.N * g.req.N;
+  const double Q3 = (double)g.req.Q * g.req.Q * g.req.Q;
+  const double geo = (g.req.op == FEMTC_OP_MASS ? 1.0
+                      : g.req.op == FEMTC_OP_STIFFNESS ? 6.0
+                                                       : 7.0) * Q3 * es;
+  double bytes = 0.0;
+  for (size_t i = 0; i < g.host.size(); ++i)
+  {
+    const double C = (double)g.host[i].num_cells;
+    bytes += C * geo;
+    bytes += g.req.gather ? 2.0 * es * (double)g.fill[i].n + 4.0 * C * N3
+                          : 2.0 * es * C * N3;
+  }
+  return bytes / 4e9;
+}
+
+// The schedule.  A group that runs one launch per problem becomes one task
+// per problem (its estimate shared out by cells), a fused group one task;
+// tasks go longest first to the side stream with the least estimated work.
+// Then the clears: the items without cells first (on b->stream), then each
+// side stream's continuous outputs, except those of assembled-scatter
+// tasks, whose second pass overwrites y.  Synchronous; called whenever a
+// group's mode changes.
+int schedule(femtc_batch* b)
+{
+  std::vector<std::pair<double, Task>> all;
+  for (size_t g = 0; g < b->groups.size(); ++g)
+  {
+    const Group& grp = b->groups[g];
+    const double est = group_estimate(grp);
+    if (grp.per_problem >= 1 && grp.per_problem <= 3 && grp.cells > 0)
+      for (size_t i = 0; i < grp.host.size(); ++i)
+        all.push_back({est * (double)grp.host[i].num_cells / (double)grp.cells,
+                       Task{(int)g, (int)i}});
+    else
+      all.push_back({est, Task{(int)g, -1}});
+  }
+  std::stable_sort(all.begin(), all.end(),
+                   [](const std::pair<double, Task>& a,
+                      const std::pair<double, Task>& c) {
+                     return a.first > c.first;
+                   });
+  double load[NSTREAMS] = {};
+  for (int i = 0; i < NSTREAMS; ++i)
+    b->tasks[i].clear();
+  for (const auto& e : all)
+  {
+    int k = 0;
+    for (int i = 1; i < NSTREAMS; ++i)
+      if (load[i] < load[k])
+        k = i;
+    b->tasks[k].push_back(e.second);
+    load[k] += e.first;
+  }
+  std::vector<FillDesc> fills = b->empty_fill;
+  b->max_empty_fill = 0;
+  for (const FillDesc& f : fills)
+    b->max_empty_fill = std::max(b->max_empty_fill, f.n);
+  for (int i = 0; i < NSTREAMS; ++i)
+  {
+    b->sfill_first[i] = fills.size();
+    b->sfill_max[i] = 0;
+    for (const Task& t : b->tasks[i])
+    {
+      const Group& grp = b->groups[t.g];
+      if (grp.per_problem == 2 || grp.fill.empty())
+        continue;
+      const size_t lo = t.item < 0 ? 0 : (size_t)t.item;
+      const size_t hi = t.item < 0 ? grp.fill.size() : lo + 1;
+      for (size_t j = lo; j < hi; ++j)
+      {
+        fills.push_back(grp.fill[j]);
+        b->sfill_max[i] = std::max(b->sfill_max[i], grp.fill[j].n);
+      }
+    }
+    b->sfill_n[i] = (int)(fills.size() - b->sfill_first[i]);
+  }
+  return upload(b, fills);
 }
 
 } // namespace
@@ -541,7 +657,7 @@ int femtc_batch_create(femtc_batch** out, const femtc_batch_item* items,
   if (st == FEMTC_OK && ncont > 65535)
     st = fail(FEMTC_ERR_INVALID_ARGUMENT, "at most 65535 continuous items");
   if (st == FEMTC_OK)
-    st = upload_fills(b);
+    st = schedule(b);
   for (int i = 0; i < NSTREAMS && st == FEMTC_OK; ++i)
     if (hipStreamCreateWithFlags(&b->side[i], hipStreamNonBlocking) !=
             hipSuccess ||
@@ -623,13 +739,17 @@ int femtc_batch_autotune(femtc_batch* b, int reps)
   hipEvent_t e0, e1;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
     return fail(FEMTC_ERR_HIP, "event creation failed");
-  // every group's clears on the device, so that a candidate that needs
-  // them (all but the assembled scatter) is timed with them
-  int st = upload_fills(b, true);
   // the whole batch a few times first: a GPU coming out of idle runs its
   // first launches slower, which would favour the candidates timed last
+  int st = FEMTC_OK;
   for (int i = 0; i < 4 && st == FEMTC_OK; ++i)
     st = run_all(b, b->stream);
+  // then every group's clears on the device, so that a candidate that
+  // needs them (all but the assembled scatter) is timed with them
+  if (st == FEMTC_OK && hipStreamSynchronize(b->stream) != hipSuccess)
+    st = fail(FEMTC_ERR_HIP, "batch autotune: warm-up failed");
+  if (st == FEMTC_OK)
+    st = upload_group_fills(b);
   for (auto& g : b->groups)
   {
     if (st != FEMTC_OK)
@@ -687,6 +807,7 @@ int femtc_batch_autotune(femtc_batch* b, int reps)
       break;
     g.req.per_wg = bestk;
     g.per_problem = best_pp;
+    g.est_ms = best / reps;
     if (g.per_problem != 2)
       free_lattices(g);
   }
@@ -704,7 +825,7 @@ int femtc_batch_autotune(femtc_batch* b, int reps)
     b->graph = nullptr;
   }
   if (st == FEMTC_OK)
-    st = upload_fills(b);
+    st = schedule(b);
   if (st != FEMTC_OK)
     return fail(st, "batch autotune failed");
   return FEMTC_OK;
@@ -779,6 +900,7 @@ int femtc_batch_set_group(femtc_batch* b, int g, int per_wg)
                     : per_wg == -3 ? 4
                                    : 0;
   grp.req.per_wg = per_wg > 0 ? per_wg : 1;
+  grp.est_ms = 0.0; // estimated from the bytes until timed again
   if (grp.per_problem != 2)
     free_lattices(grp);
   if (b->exec)
@@ -791,7 +913,7 @@ int femtc_batch_set_group(femtc_batch* b, int g, int per_wg)
     (void)hipGraphDestroy(b->graph);
     b->graph = nullptr;
   }
-  return upload_fills(b);
+  return schedule(b);
 }
 
 int femtc_batch_destroy(femtc_batch* b)
