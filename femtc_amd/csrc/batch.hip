@@ -371,27 +371,43 @@ void launch_fill(const FillDesc* d, int n, long long max_n, hipStream_t s)
 int run_all(femtc_batch* b, hipStream_t s)
 {
   launch_fill(b->d_fill, (int)b->empty_fill.size(), b->max_empty_fill, s);
-  if (hipEventRecord(b->ev_fork, s) != hipSuccess)
-    return fail(FEMTC_ERR_HIP, "batch: event record failed");
+  // only the side streams that have tasks are forked; with one such
+  // stream its work runs on s itself (a fork and join cost tens of
+  // microseconds in a replayed graph)
+  int used = 0;
   for (int i = 0; i < NSTREAMS; ++i)
-    if (hipStreamWaitEvent(b->side[i], b->ev_fork, 0) != hipSuccess)
-      return fail(FEMTC_ERR_HIP, "batch: stream wait failed");
+    used += b->tasks[i].empty() ? 0 : 1;
+  const bool fork = used > 1;
+  if (fork)
+  {
+    if (hipEventRecord(b->ev_fork, s) != hipSuccess)
+      return fail(FEMTC_ERR_HIP, "batch: event record failed");
+    for (int i = 0; i < NSTREAMS; ++i)
+      if (!b->tasks[i].empty() &&
+          hipStreamWaitEvent(b->side[i], b->ev_fork, 0) != hipSuccess)
+        return fail(FEMTC_ERR_HIP, "batch: stream wait failed");
+  }
   for (int i = 0; i < NSTREAMS; ++i)
   {
+    if (b->tasks[i].empty())
+      continue;
+    hipStream_t si = fork ? b->side[i] : s;
     launch_fill(b->d_fill + b->sfill_first[i], b->sfill_n[i],
-                b->sfill_max[i], b->side[i]);
+                b->sfill_max[i], si);
     for (const Task& t : b->tasks[i])
     {
       const Group& grp = b->groups[t.g];
-      const int st = t.item < 0 ? launch_group(grp, b->side[i])
-                                : launch_item(grp, (size_t)t.item, b->side[i]);
+      const int st = t.item < 0 ? launch_group(grp, si)
+                                : launch_item(grp, (size_t)t.item, si);
       if (st != FEMTC_OK)
         return fail(st, std::string("batch launch: ") +
                             hipGetErrorString(hipGetLastError()));
     }
   }
-  for (int i = 0; i < NSTREAMS; ++i)
+  for (int i = 0; fork && i < NSTREAMS; ++i)
   {
+    if (b->tasks[i].empty())
+      continue;
     if (hipEventRecord(b->ev_join[i], b->side[i]) != hipSuccess ||
         hipStreamWaitEvent(s, b->ev_join[i], 0) != hipSuccess)
       return fail(FEMTC_ERR_HIP, "batch: join failed");
