@@ -1090,6 +1090,20 @@ int sync_read(femtc_solver* S, void* host, const void* dev, size_t bytes)
   return FEMTC_OK;
 }
 
+// Host to device on the solver's stream, complete on return.  Solver
+// creation makes no call on the legacy default stream, so that a server
+// can create one solver on a slot's thread while another slot's thread
+// captures a second solver's CG batch into a graph.
+int sync_write(femtc_solver* S, void* dev, const void* host, size_t bytes)
+{
+  if (hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, S->stream) !=
+          hipSuccess ||
+      hipStreamSynchronize(S->stream) != hipSuccess)
+    return fail(FEMTC_ERR_HIP, std::string("solver: ") +
+                                   hipGetErrorString(hipGetLastError()));
+  return FEMTC_OK;
+}
+
 #define LAUNCH_CHECK()                                                        \
   do                                                                          \
   {                                                                           \
@@ -1391,13 +1405,13 @@ int femtc_solver_create(femtc_solver** out, const femtc_problem* prob,
       std::vector<double> xq(S->Q), wq(S->Q);
       femtc_gauss_rule(S->Q, xq.data(), wq.data());
       if (hipMemcpyAsync(S->tabw64, prob->tables, 8 * ntab,
-                         hipMemcpyDeviceToDevice, s) != hipSuccess ||
-          hipMemcpy(S->tabw64 + ntab, wq.data(), 8 * S->Q,
-                    hipMemcpyHostToDevice) != hipSuccess)
+                         hipMemcpyDeviceToDevice, s) != hipSuccess)
       {
         st = fail(FEMTC_ERR_HIP, "solver: table copy failed");
         break;
       }
+      if ((st = sync_write(S, S->tabw64 + ntab, wq.data(), 8 * S->Q)))
+        break;
       if ((st = femtc_geometry_cell(C, prob->J, S->cg64, FEMTC_F64, s)))
         break;
     }
@@ -1488,11 +1502,15 @@ int femtc_solver_create(femtc_solver** out, const femtc_problem* prob,
       // the matrix-core operand set, from the FP64 tables read back once
       std::vector<double> th(ntab);
       size_t ob = 0;
-      if ((st = femtc_mfma_tables_bytes(S->mfma_split, &ob)) ||
-          (st = sync_read(S, th.data(), prob->tables, 8 * ntab)) ||
+      if ((st = femtc_mfma_tables_bytes(S->mfma_split, &ob)))
+        break;
+      std::vector<unsigned char> oh(ob);
+      if ((st = sync_read(S, th.data(), prob->tables, 8 * ntab)) ||
           (st = dev_alloc(S, &S->mfma_ops, ob)) ||
-          (st = femtc_mfma_tables(prob->degree, prob->nq, S->mfma_split,
-                                  th.data(), S->mfma_ops)))
+          (st = femtc_mfma_tables_host(prob->degree, prob->nq,
+                                       S->mfma_split, th.data(),
+                                       oh.data())) ||
+          (st = sync_write(S, S->mfma_ops, oh.data(), ob)))
         break;
     }
     hipError_t he = hipGetLastError();
