@@ -840,8 +840,11 @@ int femtc_batch_autotune(femtc_batch* b, int reps)
     (void)hipGraphDestroy(b->graph);
     b->graph = nullptr;
   }
+  // the schedule and the clears for the modes the groups now have, also
+  // after a failure (d_fill holds the autotuner's layout until then)
+  const int ss = schedule(b);
   if (st == FEMTC_OK)
-    st = schedule(b);
+    st = ss;
   if (st != FEMTC_OK)
     return fail(st, "batch autotune failed");
   return FEMTC_OK;
