@@ -27,6 +27,7 @@ Beside the metric line's required fields it reports, on the same GPU:
   mass_c1    the cell-local P=3 mass apply on 32768 cells, cache-resident
              and streamed over 16 rotating buffer sets
   batch      200 uneven meshes in one batched, graph-captured launch set
+             (batch_groups, when named: each group of it alone)
   serving    a 500 req/s Poisson stream of applies and solves (p50/p99)
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--apply-only]
@@ -678,6 +679,57 @@ def bench_batch(torch, F):
     return out
 
 
+def bench_batch_groups(torch, F):
+    """Each (op, degree) group of the uneven FP64 batch run as a batch of its
+    own (autotuned, graph replays): where the batch's time goes without the
+    overlap of the side streams."""
+    import collections
+    from femtc_amd import workloads as Wk
+    dt, es = torch.float64, 8
+    spec = Wk.uneven_batch_spec(seed=0)
+    items = [Wk.structured_problem(torch, Wk.box_dims(c), P, P + 1, op, dt,
+                                   1000 + i) for i, (c, P, op) in enumerate(spec)]
+    groups = collections.defaultdict(list)
+    for it in items:
+        groups[(it["op"], it["degree"])].append(it)
+
+    def replay_ms(b):
+        b.autotune(5)
+        for _ in range(3):
+            b.graph_launch()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(10):
+            b.graph_launch()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / 10
+
+    rows, tot_ms, tot_b = [], 0.0, 0
+    for key in sorted(groups, key=lambda k: -sum(
+            Wk.algorithmic_bytes(i, es) for i in groups[k])):
+        sub = groups[key]
+        nb = sum(Wk.algorithmic_bytes(i, es) for i in sub)
+        b = F.Batch(sub)
+        ms = replay_ms(b)
+        rows.append({"op": "mass" if key[0] == F.OP_MASS else "stiffness",
+                     "degree": key[1], "items": len(sub), "GB": nb / 1e9,
+                     "ms": ms, "frac": nb / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "per_wg": b.groups()[0]["per_wg"]})
+        b.close()
+        tot_ms += ms
+        tot_b += nb
+    b = F.Batch(items)
+    whole = replay_ms(b)
+    b.close()
+    return {"groups": rows, "sum_alone_ms": tot_ms,
+            "sum_alone_frac": tot_b / (tot_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "whole_ms": whole,
+            "whole_frac": tot_b / (whole * 1e-3) / 1e9 / HBM_PEAK_GBS}
+
+
 def bench_serving(torch, F):
     """The request server: the small-mesh stream at 500 req/s, and the
     stream at full solve sizes (48^3, P=6) over a sweep of offered rates,
@@ -705,10 +757,11 @@ def main():
     ap.add_argument("--apply-only", action="store_true",
                     help="skip the solve, P=4 apply, batch and serving parts")
     ap.add_argument("--only", choices=("solve", "apply_p4_64cube",
-                                       "mass_c1", "batch", "serving",
-                                       "high_degree", "mfma_apply"),
+                                       "mass_c1", "batch", "batch_groups",
+                                       "serving", "high_degree", "mfma_apply"),
                     help="run the metric apply and this part only "
-                         "(high_degree runs only when named)")
+                         "(high_degree and batch_groups run only when "
+                         "named)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
                     help="only copy MiB three times (counter calibration)")
     a = ap.parse_args()
@@ -796,11 +849,12 @@ def main():
     if not a.apply_only:
         for key, fn in (("solve", bench_solve), ("apply_p4_64cube", bench_apply_p4),
                         ("mass_c1", bench_mass_c1), ("batch", bench_batch),
+                        ("batch_groups", bench_batch_groups),
                         ("serving", bench_serving),
                         ("mfma_apply", bench_mfma_apply),
                         ("high_degree", bench_high_degree)):
-            if (a.only and key != a.only) or (not a.only and
-                                              key == "high_degree"):
+            if (a.only and key != a.only) or (not a.only and key in (
+                    "high_degree", "batch_groups")):
                 continue
             print(f"[bench] {key}", file=sys.stderr, flush=True)
             out[key] = fn(torch, F)
