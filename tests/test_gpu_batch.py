@@ -174,6 +174,26 @@ def test_batch_structured_scatters_vs_alone_and_oracle(dtype, tol, scatter):
         _check(items, tol)
 
 
+def test_batch_mode_switches_keep_the_clears():
+    # the clears follow each mode change: assembled (no clear: the second
+    # pass overwrites y), then per-problem atomic and fused (y cleared
+    # first), each run on outputs holding stale values, eager and replayed
+    items = _items(torch.float64, seed0=950)
+    b = F.Batch(items)
+    cont = [g for g, info in enumerate(b.groups()) if info["gather"]]
+    assert cont
+    for mode in (-1, 0, 2, -1):
+        for g in cont:
+            b.set_group(g, mode)
+        for launch in (b.run, b.graph_launch):
+            for it in items:
+                it["y"].fill_(5.0)
+            torch.cuda.synchronize()
+            launch()
+            torch.cuda.synchronize()
+            _check(items, 1e-12)
+
+
 def test_batch_assembled_scatter_needs_structured_items():
     # a relabelled dofmap is not the structured numbering: its group keeps
     # the atomic scatter, the others may switch
