@@ -401,10 +401,11 @@ int femtc_plan_autotune(femtc_plan* plan, int op, int dtype, int nq,
 /* group (at most 64 problems) that runs either as one fused launch        */
 /* (each workgroup finds its problem by block index) or as one launch per  */
 /* problem, whichever femtc_batch_autotune measured faster (default: per   */
-/* problem).  Groups are spread over four forked streams; the sequence can */
-/* be captured once into a hipGraph and replayed.  Every output is         */
-/* overwritten (continuous y is cleared first).  The batch keeps the item  */
-/* pointers: the buffers must stay alive and in place.                     */
+/* problem).  The launches are spread over up to four forked streams,     */
+/* longest first (by the autotuner's times, else by the bytes); the        */
+/* sequence can be captured once into a hipGraph and replayed.  Every      */
+/* output is overwritten (continuous y is cleared first).  The batch keeps */
+/* the item pointers: the buffers must stay alive and in place.            */
 /* ---------------------------------------------------------------------- */
 typedef struct femtc_batch_item
 {
