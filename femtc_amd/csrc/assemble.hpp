@@ -17,6 +17,8 @@
 // work is a division by the compile-time degree.
 #pragma once
 
+#include <type_traits>
+
 #include "common.hpp"
 
 namespace femtc
@@ -100,34 +102,75 @@ __device__ inline int lattice_rows(const T* __restrict__ W, int row,
   return ni == 2 ? mj | (mj << 2) : mj;
 }
 
-// Sum of the cell values of global dof gk of a row (mask from lattice_rows).
-template <typename T, int P>
-__device__ inline T lattice_row_sum(const T* const (&rows)[4], int mask,
-                                    int gk, int ncz)
+// p[i] for 0 <= i < 2^32 / sizeof(T), addressed as base + 32-bit unsigned
+// byte offset: with a wave-uniform p the load takes p in scalar registers
+// and only the offset in a vector register.
+template <typename T>
+__device__ inline const T& at(const T* p, int i)
+{
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) +
+                                     (unsigned)i * (unsigned)sizeof(T));
+}
+template <typename T>
+__device__ inline T& at(T* p, int i)
+{
+  return *reinterpret_cast<T*>(reinterpret_cast<char*>(p) +
+                               (unsigned)i * (unsigned)sizeof(T));
+}
+
+// The rows that exist moved to the front (rows[0..n)), n returned: the mask
+// of lattice_rows is 1, 3, 5 or 15, so only mask 5 moves a row.
+template <typename T>
+__device__ inline int lattice_rows_front(const T* (&rows)[4], int mask)
+{
+  if (mask == 5)
+    rows[1] = rows[2];
+  return mask == 15 ? 4 : mask == 1 ? 1 : 2;
+}
+
+// Sum of the cell values of global dof gk of a row from its NR existing
+// lattice rows (lattice_rows_front), in the order place by place, the
+// shared point before the main one.  Both offsets stay inside the row
+// whatever the flags (has_prev or has_main always holds), so every load is
+// issued unconditionally and all are in flight before the first is waited
+// for; absent terms are then dropped by a select.
+template <typename T, int P, int NR>
+__device__ inline T lattice_row_sum(const T* const (&rows)[4], int gk, int ncz)
 {
   const int kq = gk / P;
   const int main = gk + kq;
   const bool has_main = kq < ncz;
   const bool has_prev = gk == kq * P && kq > 0;
-  // both offsets stay inside the row whatever the flags (has_prev or
-  // has_main always holds), and absent rows repeat an existing one, so the
-  // loads below may be issued unconditionally; unrolled over the 4 possible
-  // rows so that every load is in flight before the first is waited for
   const int ip = has_prev ? main - 1 : main;
   const int im = has_main ? main : ip;
-  T v[8];
+  T vp[NR], vm[NR];
 #pragma unroll
-  for (int u = 0; u < 4; ++u)
+  for (int u = 0; u < NR; ++u)
   {
-    const bool on = (mask >> u) & 1;
-    v[2 * u] = (on && has_prev) ? rows[u][ip] : T(0);
-    v[2 * u + 1] = (on && has_main) ? rows[u][im] : T(0);
+    vp[u] = at(rows[u], ip);
+    vm[u] = at(rows[u], im);
   }
   T s = T(0);
 #pragma unroll
-  for (int u = 0; u < 8; ++u)
-    s += v[u];
+  for (int u = 0; u < NR; ++u)
+  {
+    s += has_prev ? vp[u] : T(0);
+    s += has_main ? vm[u] : T(0);
+  }
   return s;
+}
+
+// f(integral_constant<int, NR>) for the row count n of lattice_rows_front
+// (wave-uniform, so the branch is a scalar one).
+template <typename F>
+__device__ inline void with_row_count(int n, F&& f)
+{
+  if (n == 1)
+    f(std::integral_constant<int, 1>{});
+  else if (n == 2)
+    f(std::integral_constant<int, 2>{});
+  else
+    f(std::integral_constant<int, 4>{});
 }
 
 constexpr int ASM_THREADS = 256;
@@ -146,21 +189,28 @@ __global__ void __launch_bounds__(ASM_THREADS)
 {
   if (skip && *skip)
     return;
+  // one row per wave, made wave-uniform: its index arithmetic and the
+  // lattice row pointers live in scalar registers
   const long long row =
-      ((long long)blockIdx.x * ASM_THREADS + threadIdx.x) >> 6;
+      (long long)blockIdx.x * (ASM_THREADS / 64) +
+      __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   if (row >= lattice_global_rows(L))
     return; // uniform over the wave
   const int lane = threadIdx.x & 63;
   const T* rows[4];
   const int mask = lattice_rows<T, P>(W, (int)row, L, rows);
+  const int nr = lattice_rows_front(rows, mask);
   const int NZ = L.ncz * P + 1;
   T* yr = y + row * NZ;
+  with_row_count(nr, [&](auto NRc) {
+    constexpr int NR = decltype(NRc)::value;
 #pragma unroll 2
-  for (int gk = lane; gk < NZ; gk += 64)
-  {
-    const T s = lattice_row_sum<T, P>(rows, mask, gk, L.ncz);
-    yr[gk] = accumulate ? yr[gk] + s : s;
-  }
+    for (int gk = lane; gk < NZ; gk += 64)
+    {
+      const T s = lattice_row_sum<T, P, NR>(rows, gk, L.ncz);
+      at(yr, gk) = accumulate ? at(yr, gk) + s : s;
+    }
+  });
 }
 
 template <typename T, int P>

@@ -286,22 +286,30 @@ __global__ void __launch_bounds__(ARB)
   const long long nrows = lattice_global_rows(L);
   const int NZ = L.ncz * P + 1;
   const int lane = threadIdx.x & 63;
-  for (long long row = (long long)blockIdx.x * (ARB / 64) + (threadIdx.x >> 6);
+  // the row is the same on every lane of a wave: made wave-uniform, its
+  // index arithmetic and the lattice row pointers live in scalar registers
+  const int wave0 = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  for (long long row = (long long)blockIdx.x * (ARB / 64) + wave0;
        row < nrows; row += (long long)NBLK * (ARB / 64))
   {
     const V* rows[4];
     const int mask = lattice_rows<V, P>(W, (int)row, L, rows);
-    const long long base = row * NZ;
+    const int nr = lattice_rows_front(rows, mask);
+    V* rr = r + row * NZ;
+    const V* dr = dinv + row * NZ;
+    with_row_count(nr, [&](auto NRc) {
+      constexpr int NR = decltype(NRc)::value;
 #pragma unroll 2
-    for (int gk = lane; gk < NZ; gk += 64)
-    {
-      const long long i = base + gk;
-      const V ri = r[i] - a * lattice_row_sum<V, P>(rows, mask, gk, L.ncz);
-      r[i] = ri;
-      const V zi = dinv[i] * ri;
-      s[0] += (double)ri * (double)zi;
-      s[1] += (double)ri * (double)ri;
-    }
+      for (int gk = lane; gk < NZ; gk += 64)
+      {
+        const V ri =
+            at(rr, gk) - a * lattice_row_sum<V, P, NR>(rows, gk, L.ncz);
+        at(rr, gk) = ri;
+        const V zi = at(dr, gk) * ri;
+        s[0] += (double)ri * (double)zi;
+        s[1] += (double)ri * (double)ri;
+      }
+    });
   }
   store_partials<2, ARB>(s, part);
 }
