@@ -47,7 +47,7 @@ size_t solver_bytes(const femtc_problem& p, int inner)
     b += vs * C * N3;
   if (inner != FEMTC_F64)
   {
-    b += (4 + 2 + 4) * n;                  // r, D^-1 (bfloat16 and FP32)
+    b += (4 + 2) * n;                      // r, D^-1 (bfloat16)
     if (p.J)
       b += 4 * CELLGEO * C;                // per-cell geometry, FP32
     else

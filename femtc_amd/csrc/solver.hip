@@ -120,9 +120,8 @@ __device__ inline void combine_partials(const double* part, double (&v)[NV])
 // condition number of the preconditioned operator by under 0.4 %, and the
 // outer FP64 loop fixes the accuracy of the result.  The streaming passes
 // then read 2 bytes per dof for it instead of 4 (the p-update at 289^3
-// dofs: 95 -> 84 us).  The assembly update reads the same rounded values
-// from an FP32 copy: in its row-wise pattern the 2-byte loads made the pass
-// take 1.5 times as long.
+// dofs: 95 -> 84 us; the refined solve with the assembly update reading
+// them too: 107.2 -> 105.3 ms).
 template <typename V>
 struct Precond
 {
@@ -275,7 +274,8 @@ constexpr int ARB = 512;
 template <typename V, int P>
 __global__ void __launch_bounds__(ARB)
     k_assemble_update(const V* __restrict__ W, V* __restrict__ r,
-                      const V* __restrict__ dinv, LatticeGrid L,
+                      const typename Precond<V>::T* __restrict__ dinv,
+                      LatticeGrid L,
                       const PcgState* __restrict__ st,
                       double* __restrict__ part)
 {
@@ -296,7 +296,7 @@ __global__ void __launch_bounds__(ARB)
     const int mask = lattice_rows<V, P>(W, (int)row, L, rows);
     const int nr = lattice_rows_front(rows, mask);
     V* rr = r + row * NZ;
-    const V* dr = dinv + row * NZ;
+    const typename Precond<V>::T* dr = dinv + row * NZ;
     with_row_count(nr, [&](auto NRc) {
       constexpr int NR = decltype(NRc)::value;
 #pragma unroll 2
@@ -305,7 +305,7 @@ __global__ void __launch_bounds__(ARB)
         const V ri =
             at(rr, gk) - a * lattice_row_sum<V, P, NR>(rows, gk, L.ncz);
         at(rr, gk) = ri;
-        const V zi = at(dr, gk) * ri;
+        const V zi = (V)precond_value(at(dr, gk)) * ri;
         s[0] += (double)ri * (double)zi;
         s[1] += (double)ri * (double)ri;
       }
@@ -431,18 +431,16 @@ __global__ void __launch_bounds__(RB)
 }
 
 // D^-1 for the FP32 inner solve: FP64 -> FP32 -> bfloat16, rounded to
-// nearest even (the values are finite: k_invert writes 1/d or 0), and the
-// same rounded values as FP32
+// nearest even (the values are finite: k_invert writes 1/d or 0)
 __global__ void __launch_bounds__(RB)
     k_precond_bf16(const double* __restrict__ x, uint16_t* __restrict__ y,
-                   float* __restrict__ yf, int64_t n)
+                   int64_t n)
 {
   GRID_LOOP(i, n)
   {
     const uint32_t u = __float_as_uint((float)x[i]);
     const uint32_t h = (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
     y[i] = (uint16_t)h;
-    yf[i] = __uint_as_float(h << 16);
   }
 }
 
@@ -634,7 +632,6 @@ struct femtc_solver
   void* p = nullptr;
   void* Ap = nullptr;
   void* dinv = nullptr;   // D^-1 as Precond<V>::T
-  void* dinv_v = nullptr; // the same values as V (the assembly pass)
   void* tab = nullptr;    // inner-precision tables and geometry
   void* G = nullptr;     // per point, or per cell (inner_affine)
   void* detJw = nullptr;
@@ -730,7 +727,6 @@ void destroy(femtc_solver* S)
   {
     dev_free(S->r);
     dev_free(S->dinv);
-    dev_free(S->dinv_v);
     dev_free(S->tab);
     dev_free(S->G);
     dev_free(S->detJw);
@@ -880,7 +876,7 @@ int launch_update(femtc_solver* S, hipStream_t s)
   {
     const V* W = (const V*)S->lattice;
     V* r = (V*)S->r;
-    const V* dinv = (const V*)S->dinv_v;
+    const auto* dinv = (const typename Precond<V>::T*)S->dinv;
     const LatticeGrid L = lattice_grid(S);
     const PcgState* st = S->st;
     switch (L.P)
@@ -1455,7 +1451,6 @@ int femtc_solver_create(femtc_solver** out, const femtc_problem* prob,
     {
       S->r = S->r64;
       S->dinv = S->dinv64;
-      S->dinv_v = S->dinv64;
       S->tab = aff ? (void*)S->tabw64 : (void*)prob->tables;
       S->G = aff ? (void*)S->cg64 : (void*)prob->G;
       S->detJw = aff ? nullptr : (void*)prob->detJw;
@@ -1464,13 +1459,11 @@ int femtc_solver_create(femtc_solver** out, const femtc_problem* prob,
     {
       // FP32 inner operator on affine cells: 7 values per cell
       if ((st = dev_alloc(S, &S->r, 4 * n)) || (st = dev_alloc(S, &S->dinv, 2 * n)) ||
-          (st = dev_alloc(S, &S->dinv_v, 4 * n)) ||
           (st = dev_alloc(S, &S->tab, 4 * (ntab + S->Q))) ||
           (st = dev_alloc(S, &S->G, 4 * CELLGEO * C)))
         break;
       hipLaunchKernelGGL(k_precond_bf16, dim3(NBLK), dim3(RB), 0, s,
-                         (const double*)S->dinv64, (uint16_t*)S->dinv,
-                         (float*)S->dinv_v, n);
+                         (const double*)S->dinv64, (uint16_t*)S->dinv, n);
       hipLaunchKernelGGL((k_convert<double, float>), dim3(NBLK), dim3(RB), 0,
                          s, (const double*)S->tabw64, (float*)S->tab,
                          ntab + S->Q);
@@ -1481,12 +1474,10 @@ int femtc_solver_create(femtc_solver** out, const femtc_problem* prob,
     else
     {
       if ((st = dev_alloc(S, &S->r, 4 * n)) || (st = dev_alloc(S, &S->dinv, 2 * n)) ||
-          (st = dev_alloc(S, &S->dinv_v, 4 * n)) ||
           (st = dev_alloc(S, &S->tab, 4 * 2 * S->N * S->Q)))
         break;
       hipLaunchKernelGGL(k_precond_bf16, dim3(NBLK), dim3(RB), 0, s,
-                         (const double*)S->dinv64, (uint16_t*)S->dinv,
-                         (float*)S->dinv_v, n);
+                         (const double*)S->dinv64, (uint16_t*)S->dinv, n);
       hipLaunchKernelGGL((k_convert<double, float>), dim3(NBLK), dim3(RB), 0,
                          s, prob->tables, (float*)S->tab,
                          (int64_t)2 * S->N * S->Q);
